@@ -1,0 +1,209 @@
+/*
+ * pdes.h -- C ABI of the MI355X (gfx950) Euler residual / Newton-Krylov library.
+ *
+ * Every entry point returns an int status: 0 on success, a negative PDES_ERR_*
+ * code on failure; pdes_last_error() returns a message for the calling thread.
+ * Handles are opaque.  All array arguments are plain host pointers laid out as
+ * the reference's Julia arrays (column-major), i.e. a C array whose axes are the
+ * Julia axes reversed:  q[ndof, Np, numEl] (Julia)  ==  q[numEl][Np][ndof] (C).
+ * Element, face, node and dof indices are 0-based.
+ *
+ * Threading: one context owns one HIP stream; calls on one context are not
+ * re-entrant, calls on distinct contexts may run on different host threads.
+ */
+#ifndef PDES_H
+#define PDES_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDES_OK 0
+#define PDES_ERR_ARG -1        /* bad argument / shape */
+#define PDES_ERR_HIP -2        /* HIP runtime error */
+#define PDES_ERR_NOMEM -3      /* device allocation failed */
+#define PDES_ERR_PHYS -4       /* non-positive density or pressure (checkDensity/checkPressure) */
+#define PDES_ERR_UNSUPPORTED -5
+#define PDES_ERR_NOCONV -6     /* Newton / GMRES did not converge */
+
+/* boundary condition functors (reference BCDict, src/solver/euler/bc.jl:2342-2369) */
+#define PDES_BC_ISENTROPIC_VORTEX 0   /* isentropicVortexBC  bc.jl:554-567 */
+#define PDES_BC_FREESTREAM 1          /* FreeStreamBC        bc.jl:1573-1586 */
+
+/* numerical face fluxes (reference FluxDict, src/solver/euler/flux.jl:1229-1242) */
+#define PDES_FLUX_ROE 0               /* RoeFlux    flux.jl:783-795 */
+#define PDES_FLUX_IRSLF 1             /* IRSLFFlux  flux.jl:964-977 */
+
+/* Mesh description: the fields of the reference's AbstractDGMesh that the
+ * residual reads (docs/src/interfaces.md:191-560).  Layouts (C order):
+ *   dxidx        [numEl][Np][dim(x)][dim(xi)]   (Julia dxidx[xi, x, Np, numEl])
+ *   jac          [numEl][Np]
+ *   interfaces   [numInterfaces][5]  elementL, elementR, faceL, faceR, orient
+ *   bndryfaces   [numBoundaryFaces][2] element, face
+ *   bndry_offsets[numBC+1], faces of BC i are bndry_offsets[i] .. bndry_offsets[i+1]-1
+ *   bc_type      [numBC]  PDES_BC_*
+ *   nrm_face     [numInterfaces][Nf][dim]  scaled normal, outward from elementL
+ *   nrm_bndry, coords_bndry [numBoundaryFaces][Nf][dim]
+ */
+typedef struct {
+  int32_t dim;
+  int32_t numEl;
+  int32_t numInterfaces;
+  int32_t numBoundaryFaces;
+  int32_t numBC;
+  const double* dxidx;
+  const double* jac;
+  const int32_t* interfaces;
+  const int32_t* bndryfaces;
+  const int32_t* bndry_offsets;
+  const int32_t* bc_type;
+  const double* nrm_face;
+  const double* nrm_bndry;
+  const double* coords_bndry;
+} pdes_mesh_desc;
+
+/* SBP operator tables (SummationByParts.jl fields used by the reference):
+ *   w[Np]                     H diagonal
+ *   Q[dim][Np][Np]            Q_d(i,j)
+ *   interp[stencil][Nf]       face interpolation (identity for diagonal-E)
+ *   perm[stencil][dim+1]      volume node of stencil entry s on face f
+ *   nbrperm[Nf][norient]      elementR face node matching elementL face node k
+ *   wface[Nf]                 face cubature weights (B)
+ */
+typedef struct {
+  int32_t Np;
+  int32_t Nf;
+  int32_t stencil;
+  int32_t norient;
+  int32_t diag_e;              /* 1: SparseFace (R is a node selection) */
+  const double* w;
+  const double* Q;
+  const double* interp;
+  const int32_t* perm;
+  const int32_t* nbrperm;
+  const double* wface;
+} pdes_sbp_desc;
+
+/* Physics options (ParamType fields, src/solver/euler/types.jl:57-345, and the
+ * option keys volume_integral_type / Flux_name / Volume_flux_name). */
+typedef struct {
+  double gamma;                /* 1.4 */
+  double R_gas;                /* 287.058 */
+  double Ma;                   /* freestream Mach */
+  double aoa;                  /* angle of attack, radians */
+  double rho_free;             /* 1.0 */
+  double p_free;               /* 1/gamma */
+  double vortex_Min;           /* isentropic vortex inflow Mach; 0.95 in the reference */
+  int32_t volume_integral_type;/* 1: weak form Q^T F, 2: entropy-stable split form (IR) */
+  int32_t face_flux;           /* PDES_FLUX_* */
+  int32_t check_positivity;    /* checkDensity / checkPressure, euler.jl:543-611 */
+} pdes_phys_desc;
+
+typedef struct pdes_ctx pdes_ctx;
+
+const char* pdes_last_error(void);
+int pdes_version(void);
+
+/* Context: copies mesh/SBP tables to HBM and allocates q/res element-major.
+ * Replaces the construction of EulerData_ (src/solver/euler/types.jl:401-723)
+ * with createMeshAndOperator (src/solver/common.jl:26-82). */
+int pdes_ctx_create(const pdes_mesh_desc* mesh, const pdes_sbp_desc* sbp,
+                    const pdes_phys_desc* phys, pdes_ctx** out);
+int pdes_ctx_destroy(pdes_ctx* ctx);
+int pdes_ctx_numdof(const pdes_ctx* ctx, int64_t* numdof);
+
+/* State transfer, eqn.q / eqn.res in the reference layout [ndof, Np, numEl]. */
+int pdes_set_state(pdes_ctx* ctx, const double* q_host);
+int pdes_get_state(pdes_ctx* ctx, double* q_host);
+int pdes_get_residual(pdes_ctx* ctx, double* res_host);
+
+/* Device-pointer variants: q/res already resident in HBM (numDof doubles). */
+int pdes_state_ptr(pdes_ctx* ctx, double** q_dev, double** res_dev);
+
+/* evalResidual(mesh, sbp, eqn, opts, t)  src/solver/euler/euler.jl:111-175:
+ * reads the device q, writes the device res.  Returns PDES_ERR_PHYS if
+ * check_positivity is set and a node has rho <= 0 or p <= 0. */
+int pdes_eval_residual(pdes_ctx* ctx, double t);
+
+/* calcNorm(eqn, res, strongres=true)  src/Utils/Utils.jl:427-449 on the device
+ * residual: sqrt(sum res_i^2 Minv_i). */
+int pdes_residual_norm(pdes_ctx* ctx, double* norm);
+
+/* evaldRdqProduct  src/interface2.jl:454-498: b = Im R(q + i h v)/h on the
+ * device, h = 1e-20 (epsilon, src/input/read_input.jl:477-479).  v and b are
+ * host arrays of numDof doubles. */
+int pdes_jv_complex_step(pdes_ctx* ctx, const double* v_host, double* b_host);
+
+/* Explicit Jacobian: evalJacobian  src/solver/euler/evalJacobian.jl:6-72 with
+ * the assembler of src/jacobian/jacobian.jl:703-1759 targeting an on-device
+ * BSR matrix, block size ndof, sparsity of SparseMatrixCSC(mesh, ..., INVISCID,
+ * face_type)  src/jacobian/sparse.jl:250-495.  Values are stored in FP32 (or FP64
+ * when fp64_blocks != 0). */
+int pdes_jacobian_setup(pdes_ctx* ctx, int fp64_blocks);
+int pdes_jacobian_nnzb(pdes_ctx* ctx, int64_t* nrows_blk, int64_t* nnzb);
+int pdes_get_bsr_pattern(pdes_ctx* ctx, int64_t* rowptr, int32_t* colind);
+int pdes_assemble_jacobian(pdes_ctx* ctx);
+int pdes_get_bsr_values(pdes_ctx* ctx, void* values_host);
+/* b = J x with the assembled BSR matrix (applyLinearOperator of PetscMatLO,
+ * src/linearsolvers/lo_petscmat.jl), host vectors. */
+int pdes_bsr_matvec(pdes_ctx* ctx, const double* x_host, double* b_host);
+
+/* Preconditioner (AbstractPC, src/linearsolvers/LinearSolvers.jl:185-400):
+ * kind 0 = element block-Jacobi (NewtonBDiagPC, preconditioning_types.jl:41),
+ * kind 1 = ILU(0) on the BSR matrix.  Factor and apply in FP32. */
+#define PDES_PC_NONE 0
+#define PDES_PC_BJACOBI 1
+#define PDES_PC_ILU0 2
+int pdes_pc_setup(pdes_ctx* ctx, int kind);
+int pdes_pc_apply(pdes_ctx* ctx, const double* b_host, double* x_host);
+
+/* GMRES(restart) with right preconditioning and modified Gram-Schmidt
+ * (PETSc KSP options of src/input/read_input.jl:560-581) on the device.
+ * mode 0: assembled BSR matvec, mode 1: matrix-free complex-step Jv. */
+int pdes_gmres(pdes_ctx* ctx, const double* b_host, double* x_host, double rtol,
+               double atol, int restart, int maxit, int mode, int* iters,
+               double* final_resnorm);
+
+/* Newton (newtonInner, src/NonlinearSolvers/newton.jl:137-304) on the device
+ * state: J dq = -R, q += dq until ||R||_{M^-1} <= res_abstol or reltol. */
+typedef struct {
+  double res_abstol;
+  double res_reltol;
+  double step_tol;
+  int32_t itermax;
+  double krylov_reltol;        /* inexact-Newton forcing, default 1e-2 */
+  double krylov_abstol;
+  int32_t krylov_itermax;
+  int32_t gmres_restart;       /* 30 */
+  int32_t pc_kind;             /* PDES_PC_* */
+  int32_t fp64_blocks;         /* store Jacobian/PC in FP64 instead of FP32 */
+  int32_t jv_mode;             /* 0 BSR, 1 matrix-free complex step */
+} pdes_newton_opts;
+
+typedef struct {
+  int32_t iterations;
+  int32_t converged;
+  int32_t total_krylov_iters;
+  double res_norm0;
+  double res_norm;
+  double step_norm;
+  double elapsed_s;
+} pdes_newton_report;
+
+int pdes_newton_solve(pdes_ctx* ctx, const pdes_newton_opts* opts,
+                      pdes_newton_report* report);
+
+/* Boundary force functionals (calcBndryFunctional,
+ * src/solver/euler/boundary_functional.jl:82-126):
+ * kind 0 = force vector (dim values), 1 = lift, 2 = drag, 3 = entropy flux,
+ * 4 = mass flow; over the BC groups listed in bcnums. */
+int pdes_eval_functional(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
+                         double* val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

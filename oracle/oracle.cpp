@@ -1,0 +1,369 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY: the CPU parity oracle.
+//
+// An FP64 CPU restatement of the reference's Euler residual path, used by
+// tests/, the smoke check and bench.py's CPU-baseline leg to check and time the
+// HIP library.  It is never linked into, or called by, the product library.
+//
+// Restated reference functions (file:line, relative to the reference root):
+//   evalResidual            src/solver/euler/euler.jl:111-175
+//   dataPrep                src/solver/euler/euler.jl:441-519
+//   getEulerFlux            src/solver/euler/euler_funcs.jl:23-58
+//   evalVolumeIntegrals     src/solver/euler/euler.jl:628-658 (weakdifferentiate!, trans=true)
+//   calcVolumeIntegralsSplitFormLinear  src/solver/euler/euler_funcs.jl:240-288
+//   interpolateFace / calcFaceFlux      src/solver/euler/flux.jl:613-641, 37-64
+//   calcFaceIntegral_nopre  src/solver/euler/flux.jl:79-125
+//   evalFaceIntegrals       src/solver/euler/euler.jl:770-802 (interiorfaceintegrate!, Subtract)
+//   interpolateBoundary / getBCFluxes / calcBoundaryFlux  src/solver/euler/bc.jl:162-175, 49-80, 251-284
+//   evalBoundaryIntegrals   src/solver/euler/euler.jl:669-690 (boundaryintegrate!, Subtract)
+//   checkDensity / checkPressure  src/solver/euler/euler.jl:543-611
+//   calcJacobianSparse (complex step, distance-2 coloring)  src/jacobian/jacobian.jl:365-446
+//   SparseMatrixCSC(mesh, Tv, INVISCID, face_type)           src/jacobian/sparse.jl:250-495
+// The SBP application routines (weakdifferentiate!, interiorfaceinterpolate!,
+// interiorfaceintegrate!, boundaryinterpolate!, boundaryintegrate!) live in
+// the external SummationByParts.jl; they are restated from their documented
+// definitions (res += Q_d^T F_d; u_face = R u; res -= R^T B F).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../include/pdes.h"
+#include "euler_oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+
+Params make_params(const pdes_phys_desc* ph, int dim) {
+  Params p;
+  p.dim = dim;
+  p.gamma = ph->gamma;
+  p.gamma_1 = ph->gamma - 1;
+  p.R = ph->R_gas;
+  p.cv = p.R / p.gamma_1;
+  p.Ma = ph->Ma;
+  p.aoa = ph->aoa;
+  p.rho_free = ph->rho_free;
+  p.p_free = ph->p_free;
+  p.E_free = ph->p_free / p.gamma_1 + 0.5 * ph->rho_free * ph->Ma * ph->Ma;  // types.jl:252
+  p.vortex_Min = ph->vortex_Min;
+  return p;
+}
+
+// accumulate helper: in |.| mode the magnitude of each contribution is summed
+// (gives the floating-point scale of every residual entry)
+template <class T> inline void acc(T& dst, const T& v, bool absmode) {
+  if (absmode) dst += T(std::abs(v)); else dst += v;
+}
+
+template <class T>
+int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+             const T* q, T* res, bool absmode) {
+  const int dim = m->dim, nd = dim + 2, Np = s->Np, Nf = s->Nf, nfaces = dim + 1;
+  const int numEl = m->numEl;
+  Params p = make_params(ph, dim);
+  const size_t ndofs = (size_t)numEl * Np * nd;
+  for (size_t i = 0; i < ndofs; ++i) res[i] = 0.0;
+
+  // checkDensity / checkPressure (euler.jl:543-611)
+  if (ph->check_positivity && !absmode) {
+    for (size_t n = 0; n < (size_t)numEl * Np; ++n) {
+      const T* qn = q + n * nd;
+      if (re(qn[0]) <= 0) return PDES_ERR_PHYS;
+      if (re(calcPressure(p, qn)) <= 0) return PDES_ERR_PHYS;
+    }
+  }
+  auto dxidx = [&](int e, int j, int x, int xi) { return m->dxidx[(((size_t)e * Np + j) * dim + x) * dim + xi]; };
+  auto Q = [&](int d, int i, int j) { return s->Q[((size_t)d * Np + i) * Np + j]; };
+
+  // ---- volume integrals
+  if (ph->volume_integral_type == 1) {
+    std::vector<T> F((size_t)Np * dim * nd);
+    for (int e = 0; e < numEl; ++e) {
+      // getEulerFlux: flux_parametric[:, j, e, d] with nrm = dxidx[d, :, j, e]
+      for (int j = 0; j < Np; ++j)
+        for (int d = 0; d < dim; ++d) {
+          double nrm[3];
+          for (int x = 0; x < dim; ++x) nrm[x] = dxidx(e, j, x, d);
+          calcEulerFlux(p, q + ((size_t)e * Np + j) * nd, nrm, &F[((size_t)j * dim + d) * nd]);
+        }
+      // weakdifferentiate!(sbp, d, F_d, res, trans=true): res[:, i] += sum_j Q_d[j, i] F_d[:, j]
+      for (int d = 0; d < dim; ++d)
+        for (int i = 0; i < Np; ++i)
+          for (int j = 0; j < Np; ++j) {
+            double qji = Q(d, j, i);
+            for (int k = 0; k < nd; ++k)
+              acc(res[((size_t)e * Np + i) * nd + k], qji * F[((size_t)j * dim + d) * nd + k], absmode);
+          }
+    }
+  } else {
+    // calcVolumeIntegralsSplitFormLinear with IRFlux (multi-direction)
+    std::vector<double> S((size_t)Np * Np * dim);
+    for (int d = 0; d < dim; ++d)
+      for (int i = 0; i < Np; ++i)
+        for (int j = 0; j < Np; ++j) S[((size_t)i * Np + j) * dim + d] = 0.5 * (Q(d, i, j) - Q(d, j, i));
+    T Fd[15];
+    double nrmD[9];
+    for (int e = 0; e < numEl; ++e)
+      for (int j = 0; j < Np; ++j) {
+        const T* qj = q + ((size_t)e * Np + j) * nd;
+        for (int k = 0; k < j; ++k) {
+          const T* qk = q + ((size_t)e * Np + k) * nd;
+          // nrmD[p, d] = dxidx[d, p, j, i]
+          for (int d = 0; d < dim; ++d)
+            for (int x = 0; x < dim; ++x) nrmD[x * dim + d] = dxidx(e, j, x, d);
+          calcEulerFlux_IR(p, qj, qk, nrmD, dim, Fd);
+          for (int d = 0; d < dim; ++d) {
+            double sjk = S[((size_t)j * Np + k) * dim + d];
+            for (int pp = 0; pp < nd; ++pp) {
+              T v = 2 * sjk * Fd[d * nd + pp];
+              acc(res[((size_t)e * Np + j) * nd + pp], T(-v), absmode);
+              acc(res[((size_t)e * Np + k) * nd + pp], v, absmode);
+            }
+          }
+        }
+      }
+  }
+
+  // ---- boundary integrals (precompute_boundary_flux: getBCFluxes then boundaryintegrate!)
+  {
+    std::vector<T> qb((size_t)Nf * nd), fb((size_t)Nf * nd);
+    for (int bc = 0; bc < m->numBC; ++bc) {
+      for (int b = m->bndry_offsets[bc]; b < m->bndry_offsets[bc + 1]; ++b) {
+        int e = m->bndryfaces[2 * b], f = m->bndryfaces[2 * b + 1];
+        // boundaryinterpolate!
+        for (int kf = 0; kf < Nf; ++kf)
+          for (int k = 0; k < nd; ++k) {
+            T v = 0.0;
+            for (int st = 0; st < s->stencil; ++st)
+              v += s->interp[st * Nf + kf] * q[((size_t)e * Np + s->perm[st * nfaces + f]) * nd + k];
+            qb[kf * nd + k] = v;
+          }
+        for (int kf = 0; kf < Nf; ++kf) {
+          const double* coords = m->coords_bndry + ((size_t)b * Nf + kf) * dim;
+          const double* nrm = m->nrm_bndry + ((size_t)b * Nf + kf) * dim;
+          double qg_d[5];
+          if (m->bc_type[bc] == PDES_BC_ISENTROPIC_VORTEX) calcIsentropicVortex(p, coords, qg_d);
+          else calcFreeStream(p, qg_d);
+          T qg[5];
+          for (int k = 0; k < nd; ++k) qg[k] = qg_d[k];
+          RoeSolver(p, &qb[kf * nd], qg, nrm, &fb[kf * nd]);
+        }
+        // boundaryintegrate!(..., Subtract): res[:, perm[st,f]] -= sum_k interp[st,k] wface[k] F[:, k]
+        for (int st = 0; st < s->stencil; ++st) {
+          int node = s->perm[st * nfaces + f];
+          for (int kf = 0; kf < Nf; ++kf) {
+            double fac = s->interp[st * Nf + kf] * s->wface[kf];
+            for (int k = 0; k < nd; ++k)
+              acc(res[((size_t)e * Np + node) * nd + k], T(-fac * fb[kf * nd + k]), absmode);
+          }
+        }
+      }
+    }
+  }
+
+  // ---- face integrals
+  {
+    std::vector<T> qL((size_t)Nf * nd), qR((size_t)Nf * nd), ff((size_t)Nf * nd);
+    for (int i = 0; i < m->numInterfaces; ++i) {
+      const int* it = m->interfaces + 5 * i;
+      int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
+      // interiorfaceinterpolate!
+      for (int kf = 0; kf < Nf; ++kf) {
+        int kR = s->nbrperm[kf * s->norient + o];
+        for (int k = 0; k < nd; ++k) {
+          T vl = 0.0, vr = 0.0;
+          for (int st = 0; st < s->stencil; ++st) {
+            vl += s->interp[st * Nf + kf] * q[((size_t)eL * Np + s->perm[st * nfaces + fL]) * nd + k];
+            vr += s->interp[st * Nf + kR] * q[((size_t)eR * Np + s->perm[st * nfaces + fR]) * nd + k];
+          }
+          qL[kf * nd + k] = vl;
+          qR[kf * nd + k] = vr;
+        }
+      }
+      for (int kf = 0; kf < Nf; ++kf) {
+        const double* nrm = m->nrm_face + ((size_t)i * Nf + kf) * dim;
+        if (ph->face_flux == PDES_FLUX_ROE) RoeSolver(p, &qL[kf * nd], &qR[kf * nd], nrm, &ff[kf * nd]);
+        else calcEulerFlux_IRSLF(p, &qL[kf * nd], &qR[kf * nd], nrm, &ff[kf * nd]);
+      }
+      // interiorfaceintegrate!(..., Subtract)
+      for (int st = 0; st < s->stencil; ++st) {
+        int nodeL = s->perm[st * nfaces + fL];
+        int nodeR = s->perm[st * nfaces + fR];
+        for (int kf = 0; kf < Nf; ++kf) {
+          int kR = s->nbrperm[kf * s->norient + o];
+          double facL = s->interp[st * Nf + kf] * s->wface[kf];
+          double facR = s->interp[st * Nf + kR] * s->wface[kR];
+          for (int k = 0; k < nd; ++k) {
+            acc(res[((size_t)eL * Np + nodeL) * nd + k], T(-facL * ff[kf * nd + k]), absmode);
+            acc(res[((size_t)eR * Np + nodeR) * nd + k], T(facR * ff[kf * nd + k]), absmode);
+          }
+        }
+      }
+    }
+  }
+  return PDES_OK;
+}
+
+// element adjacency through interfaces
+std::vector<std::vector<int>> element_neighbors(const pdes_mesh_desc* m) {
+  std::vector<std::vector<int>> nb(m->numEl);
+  for (int i = 0; i < m->numInterfaces; ++i) {
+    nb[m->interfaces[5 * i]].push_back(m->interfaces[5 * i + 1]);
+    nb[m->interfaces[5 * i + 1]].push_back(m->interfaces[5 * i]);
+  }
+  return nb;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oracle_residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                    const double* q, double* res) {
+  return residual<double>(m, s, ph, q, res, false);
+}
+
+// per-entry floating-point scale: the residual with every contribution replaced by its magnitude
+int oracle_residual_scale(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                          const double* q, double* res) {
+  return residual<double>(m, s, ph, q, res, true);
+}
+
+int oracle_residual_complex(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                            const double* q_re, const double* q_im, double* res_re, double* res_im) {
+  size_t n = (size_t)m->numEl * s->Np * (m->dim + 2);
+  std::vector<cplx> q(n), r(n);
+  for (size_t i = 0; i < n; ++i) q[i] = cplx(q_re[i], q_im[i]);
+  int st = residual<cplx>(m, s, ph, q.data(), r.data(), false);
+  for (size_t i = 0; i < n; ++i) { res_re[i] = r[i].real(); res_im[i] = r[i].imag(); }
+  return st;
+}
+
+// evaldRdqProduct (src/interface2.jl:454-498): b = Im R(q + i h v) / h
+int oracle_jv_complex_step(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                           const double* q, const double* v, double h, double* b) {
+  size_t n = (size_t)m->numEl * s->Np * (m->dim + 2);
+  std::vector<cplx> qc(n), r(n);
+  for (size_t i = 0; i < n; ++i) qc[i] = cplx(q[i], h * v[i]);
+  int st = residual<cplx>(m, s, ph, qc.data(), r.data(), false);
+  for (size_t i = 0; i < n; ++i) b[i] = r[i].imag() / h;
+  return st;
+}
+
+// Block sparsity of SparseMatrixCSC(mesh, Tv, INVISCID, face_type) (sparse.jl:250-495),
+// node level (block size ndof).  First call with colind == NULL returns
+// rowptr only (nrows+1 entries); second call fills sorted column lists.
+int oracle_sparsity_bsr(const pdes_mesh_desc* m, const pdes_sbp_desc* s, int64_t* rowptr, int32_t* colind) {
+  const int Np = s->Np, Nf = s->Nf, nfaces = m->dim + 1;
+  const int64_t nrows = (int64_t)m->numEl * Np;
+  std::vector<std::vector<int32_t>> rows(nrows);
+  // volume terms: every node of an element couples to all nodes of that element
+  for (int e = 0; e < m->numEl; ++e)
+    for (int j = 0; j < Np; ++j)
+      for (int pp = 0; pp < Np; ++pp) rows[(int64_t)e * Np + j].push_back(e * Np + pp);
+  // interface terms
+  for (int i = 0; i < m->numInterfaces; ++i) {
+    const int* it = m->interfaces + 5 * i;
+    int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
+    if (!s->diag_e) {  // face_type 1: all permL <-> all permR
+      for (int j = 0; j < s->stencil; ++j) {
+        int nL = s->perm[j * nfaces + fL], nR = s->perm[j * nfaces + fR];
+        for (int pp = 0; pp < s->stencil; ++pp) {
+          rows[(int64_t)eL * Np + nL].push_back(eR * Np + s->perm[pp * nfaces + fR]);
+          rows[(int64_t)eR * Np + nR].push_back(eL * Np + s->perm[pp * nfaces + fL]);
+        }
+      }
+    } else {  // face_type 2: node <-> nbrperm partner
+      for (int j = 0; j < Nf; ++j) {
+        int nodeL = s->perm[j * nfaces + fL];
+        int pnbr = s->nbrperm[j * s->norient + o];
+        int nodeR = s->perm[pnbr * nfaces + fR];
+        rows[(int64_t)eL * Np + nodeL].push_back(eR * Np + nodeR);
+        rows[(int64_t)eR * Np + nodeR].push_back(eL * Np + nodeL);
+      }
+    }
+  }
+  rowptr[0] = 0;
+  for (int64_t r = 0; r < nrows; ++r) rowptr[r + 1] = rowptr[r] + (int64_t)rows[r].size();
+  if (colind) {
+    for (int64_t r = 0; r < nrows; ++r) {
+      std::sort(rows[r].begin(), rows[r].end());
+      std::memcpy(colind + rowptr[r], rows[r].data(), rows[r].size() * sizeof(int32_t));
+    }
+  }
+  return PDES_OK;
+}
+
+// Distance-2 element coloring (greedy, element order): no two elements of a
+// color share an element-face neighbour, so one complex residual per
+// (color, node, dof) separates every column.
+int oracle_color_elements(const pdes_mesh_desc* m, int32_t* color, int32_t* ncolors) {
+  auto nb = element_neighbors(m);
+  int nc = 0;
+  std::vector<int> mark;
+  for (int e = 0; e < m->numEl; ++e) {
+    mark.assign(nc + 1, 0);
+    auto visit = [&](int x) { if (x < e) mark[color[x]] = 1; };
+    for (int a : nb[e]) {
+      visit(a);
+      for (int b : nb[a]) if (b != e) visit(b);
+    }
+    int c = 0;
+    while (c < nc && mark[c]) ++c;
+    color[e] = c;
+    if (c == nc) ++nc;
+  }
+  *ncolors = nc;
+  return PDES_OK;
+}
+
+// calcJacobianSparse (jacobian.jl:365-446): complex-step Jacobian through
+// element coloring, values scattered into the BSR pattern from
+// oracle_sparsity_bsr (block = [ndof][ndof] row-major, row = residual dof).
+int oracle_jacobian_cs(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                       const double* q, const int64_t* rowptr, const int32_t* colind, double* val) {
+  const int nd = m->dim + 2, Np = s->Np;
+  const size_t n = (size_t)m->numEl * Np * nd;
+  const double h = 1e-20;
+  std::vector<int32_t> color(m->numEl);
+  int32_t nc = 0;
+  oracle_color_elements(m, color.data(), &nc);
+  auto nb = element_neighbors(m);
+  std::fill(val, val + (size_t)rowptr[(size_t)m->numEl * Np] * nd * nd, 0.0);
+  std::vector<cplx> qc(n), r(n);
+  for (int c = 0; c < nc; ++c)
+    for (int j = 0; j < Np; ++j)
+      for (int k = 0; k < nd; ++k) {
+        for (size_t i = 0; i < n; ++i) qc[i] = q[i];
+        for (int e = 0; e < m->numEl; ++e)
+          if (color[e] == c) qc[((size_t)e * Np + j) * nd + k] += cplx(0, h);
+        int st = residual<cplx>(m, s, ph, qc.data(), r.data(), false);
+        if (st) return st;
+        for (int e = 0; e < m->numEl; ++e) {
+          if (color[e] != c) continue;
+          int32_t col = e * Np + j;
+          std::vector<int> rowsel = nb[e];
+          rowsel.push_back(e);
+          for (int er : rowsel)
+            for (int i = 0; i < Np; ++i) {
+              int64_t row = (int64_t)er * Np + i;
+              const int32_t* b = std::lower_bound(colind + rowptr[row], colind + rowptr[row + 1], col);
+              bool present = (b != colind + rowptr[row + 1] && *b == col);
+              for (int kr = 0; kr < nd; ++kr) {
+                double d = r[((size_t)er * Np + i) * nd + kr].imag() / h;
+                if (!present) {
+                  if (d != 0.0) return PDES_ERR_ARG;   // entry outside the sparsity pattern
+                  continue;
+                }
+                val[((size_t)(b - colind) * nd + kr) * nd + k] = d;
+              }
+            }
+        }
+      }
+  return PDES_OK;
+}
+
+}  // extern "C"
