@@ -1,0 +1,156 @@
+"""Headline benchmark: DOF-residual evaluations per second of the Euler DG-SBP
+residual on the 256x256 quarter-annulus isentropic-vortex problem (131,072
+triangles), SBP diagonal-E at p=2 and p=4, with (a) Roe face flux and weak-form
+Q^T F volume integral and (b) Ismail-Roe entropy-stable split form with the
+IRSLF face flux.  A step is one classical RK4 step (4 residual evaluations plus
+the Minv-scaled stage updates) of each of the four variants; the default run
+times 250 steps, i.e. 1,000 residual evaluations per variant.
+
+    python bench.py [--gpus 1] [--steps 250] [--warmup 5]
+
+Prints one JSON line.  The state is resident in HBM before the timed region.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "pdesolver.jl_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def residual_bytes_per_launch(m, op):
+    """Algorithmic HBM bytes of one residual launch: q read + res write +
+    dxidx read (per node), the element->face records, the face normals (once
+    per interface) and the boundary face data, each touched once."""
+    Np, Nf, dim, nd = op.numnodes, op.face.numnodes, m.dim, m.dim + 2
+    per_el = 8 * (nd * Np + nd * Np + dim * dim * Np) + 16 * (dim + 1)
+    faces = 8 * dim * Nf * m.numInterfaces + 8 * 2 * dim * Nf * m.numBoundaryFaces
+    return per_el * m.numEl + faces
+
+
+def variants():
+    return [(2, "roe"), (4, "roe"), (2, "ir"), (4, "ir")]
+
+
+def make(p, flux):
+    from pdes_amd import problems
+    m, op, ph = problems.make_problem("vortex", 256, p, "diage", flux)
+    q = problems.vortex_fourier_state(m, ph, seed=1, amp=1e-2)
+    return m, op, ph, q
+
+
+def dt_for(m, q, ph, p):
+    # dt = 0.05 h / ((|u| + a) (2p + 1)) with h the smallest element size
+    h = np.sqrt(2.0 * (1.0 / m.jac[:, 0]).min() * 2.0)
+    rho = q[..., 0]
+    u = np.sqrt(q[..., 1] ** 2 + q[..., 2] ** 2) / rho
+    pr = (ph.gamma - 1) * (q[..., 3] - 0.5 * rho * u * u)
+    a = np.sqrt(ph.gamma * pr / rho)
+    return 0.05 * h / ((u + a).max() * (2 * p + 1))
+
+
+def cpu_baseline(budget_s=12.0):
+    """The CPU oracle (a serial C++ restatement of the reference residual)
+    timed on this host, one thread, on a bounded sample: residual evaluations
+    of each variant until the per-variant time budget is spent."""
+    import oracle_py
+    from pdes_amd import abi
+    oracle_py.build()
+    total_dof = 0
+    total_t = 0.0
+    sample = []
+    for p, flux in variants():
+        m, op, ph, q = make(p, flux)
+        b = abi.make_descriptors(m, op, ph)
+        n = 0
+        t0 = time.perf_counter()
+        while True:
+            oracle_py.residual(b, q)
+            n += 1
+            if time.perf_counter() - t0 > budget_s / 4:
+                break
+        dt = time.perf_counter() - t0
+        total_dof += q.size * n
+        total_t += dt
+        sample.append(f"p{p}-{flux}:{n}")
+    return {"value": total_dof / total_t, "unit": "DOF-residual evals/s", "cores": 1, "kind": "port",
+            "sample": "full 256x256 mesh, residual evaluations per variant " + ",".join(sample)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=250)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X; "
+              "--gpus above 1 is not supported", file=sys.stderr)
+        sys.exit(2)
+    from pdes_amd import euler
+
+    results = {}
+    total_dof_evals = 0
+    total_ms = 0.0
+    kern = {}
+    for p, flux in variants():
+        m, op, ph, q = make(p, flux)
+        eqn = euler.EulerData(m, op, ph)
+        eqn.set_q(q)
+        h = dt_for(m, q, ph, p)
+        euler.rk4(eqn, args.warmup, h)
+        eqn.set_q(q)
+        ms = euler.rk4(eqn, args.steps, h, timed=True)
+        dof_evals = 4 * args.steps * q.size
+        total_dof_evals += dof_evals
+        total_ms += ms
+        # residual kernel alone, HIP events on the context stream
+        kms = euler.time_residual(eqn, 50)
+        nbytes = residual_bytes_per_launch(m, op)
+        kern[(p, flux)] = (kms, nbytes)
+        results[f"p{p}_{flux}"] = {"dof_evals_per_s": dof_evals / (ms / 1e3), "ms_per_rk4_step": ms / args.steps,
+                                   "residual_kernel_ms": kms, "residual_GBs": nbytes / (kms / 1e3) / 1e9,
+                                   "numDof": int(q.size)}
+        eqn.free()
+    # dominant kernel by time in the step
+    (pd, fd), (kms, nbytes) = max(kern.items(), key=lambda kv: kv[1][0])
+    achieved = nbytes / (kms / 1e3) / 1e9
+    line = {
+        "metric": "DOF-residual evals/s",
+        "value": total_dof_evals / (total_ms / 1e3),
+        "unit": "DOF-residual evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "2D isentropic vortex, 256x256 quarter annulus (131072 triangles), "
+                               "SBP diagonal-E p=2 and p=4, Roe weak form and Ismail-Roe split form, "
+                               "residuals inside RK4 stages",
+                   "variants": results},
+        "roofline": {"kernel": f"euler_residual p={pd} {fd}", "bound": "hbm", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -367,3 +367,70 @@ int oracle_jacobian_cs(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pd
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// point functions, for the known-answer tests of the reference's test suite
+// ---------------------------------------------------------------------------
+namespace {
+Params point_params(int dim, const pdes_phys_desc* ph) { return make_params(ph, dim); }
+}
+
+extern "C" {
+
+double oracle_pressure(int dim, const pdes_phys_desc* ph, const double* q) {
+  Params p = point_params(dim, ph);
+  return calcPressure(p, q);
+}
+
+void oracle_euler_flux(int dim, const pdes_phys_desc* ph, const double* q, const double* dir, double* F) {
+  Params p = point_params(dim, ph);
+  calcEulerFlux(p, q, dir, F);
+}
+
+void oracle_roe(int dim, const pdes_phys_desc* ph, const double* qL, const double* qR, const double* nrm,
+                double* F) {
+  Params p = point_params(dim, ph);
+  RoeSolver(p, qL, qR, nrm, F);
+}
+
+void oracle_roe_complex(int dim, const pdes_phys_desc* ph, const double* qL_re, const double* qL_im,
+                        const double* qR_re, const double* qR_im, const double* nrm, double* F_re, double* F_im) {
+  Params p = point_params(dim, ph);
+  cplx qL[5], qR[5], F[5];
+  for (int i = 0; i < dim + 2; ++i) { qL[i] = cplx(qL_re[i], qL_im[i]); qR[i] = cplx(qR_re[i], qR_im[i]); }
+  RoeSolver(p, qL, qR, nrm, F);
+  for (int i = 0; i < dim + 2; ++i) { F_re[i] = F[i].real(); F_im[i] = F[i].imag(); }
+}
+
+// dir: [x][ndir]
+void oracle_ir_flux(int dim, const pdes_phys_desc* ph, const double* qL, const double* qR, const double* dir,
+                    int ndir, double* F) {
+  Params p = point_params(dim, ph);
+  calcEulerFlux_IR(p, qL, qR, dir, ndir, F);
+}
+
+void oracle_irslf_flux(int dim, const pdes_phys_desc* ph, const double* qL, const double* qR,
+                       const double* nrm, double* F) {
+  Params p = point_params(dim, ph);
+  calcEulerFlux_IRSLF(p, qL, qR, nrm, F);
+}
+
+double oracle_logavg(double aL, double aR) { return logavg(aL, aR); }
+
+void oracle_logavg_complex(double aL_re, double aL_im, double aR_re, double aR_im, double* out) {
+  cplx r = logavg(cplx(aL_re, aL_im), cplx(aR_re, aR_im));
+  out[0] = r.real();
+  out[1] = r.imag();
+}
+
+void oracle_vortex(const pdes_phys_desc* ph, const double* coords, double* sol) {
+  Params p = point_params(2, ph);
+  calcIsentropicVortex(p, coords, sol);
+}
+
+void oracle_ir_vars(int dim, const pdes_phys_desc* ph, const double* q, double* v) {
+  Params p = point_params(dim, ph);
+  convertToIR(p, q, v);
+}
+
+}  // extern "C"

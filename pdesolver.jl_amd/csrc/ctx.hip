@@ -1,0 +1,243 @@
+// ctx.hip -- context lifetime, table upload and state transfer (C ABI).
+#include <cstdarg>
+#include <cstring>
+
+#include "ctx.hpp"
+
+namespace pdes {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+template <class T>
+static int upload(T** dst, const T* src, size_t n, hipStream_t s) {
+  if (n == 0) { *dst = nullptr; return PDES_OK; }
+  hipError_t e = hipMalloc((void**)dst, n * sizeof(T));
+  if (e != hipSuccess) { set_error("hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e)); return PDES_ERR_NOMEM; }
+  if (src) PDES_HIP(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return PDES_OK;
+}
+
+int sync_check_flag(pdes_ctx* c) {
+  int32_t f = 0;
+  PDES_HIP(hipMemcpyAsync(&f, c->d_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  if (f) {
+    PDES_HIP(hipMemsetAsync(c->d_flag, 0, sizeof(int32_t), c->stream));
+    set_error("non-positive density or pressure in the state (checkDensity/checkPressure)");
+    return PDES_ERR_PHYS;
+  }
+  return PDES_OK;
+}
+
+}  // namespace pdes
+
+using namespace pdes;
+
+extern "C" {
+
+const char* pdes_last_error(void) { return g_err; }
+int pdes_version(void) { return 1; }
+
+int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                    pdes_ctx** out) {
+  PDES_CHECK(m && s && ph && out, PDES_ERR_ARG, "null argument");
+  PDES_CHECK(m->dim == 2 || m->dim == 3, PDES_ERR_ARG, "dim must be 2 or 3");
+  PDES_CHECK(m->numEl > 0 && s->Np > 0 && s->Nf > 0, PDES_ERR_ARG, "empty mesh or operator");
+  pdes_ctx* c = new pdes_ctx();
+  c->dim = m->dim;
+  c->nd = m->dim + 2;
+  c->nfaces = m->dim + 1;
+  c->numEl = m->numEl;
+  c->Np = s->Np;
+  c->Nf = s->Nf;
+  c->stencil = s->stencil;
+  c->norient = s->norient;
+  c->diag_e = s->diag_e;
+  c->numIface = m->numInterfaces;
+  c->numBndry = m->numBoundaryFaces;
+  c->numBC = m->numBC;
+  c->numDof = (int64_t)m->numEl * s->Np * c->nd;
+  c->phys = *ph;
+  c->pc.gamma = ph->gamma;
+  c->pc.gamma_1 = ph->gamma - 1;
+  c->pc.R = ph->R_gas;
+  c->pc.cv = ph->R_gas / (ph->gamma - 1);
+  c->pc.Ma = ph->Ma;
+  c->pc.aoa = ph->aoa;
+  c->pc.rho_free = ph->rho_free;
+  c->pc.E_free = ph->p_free / (ph->gamma - 1) + 0.5 * ph->rho_free * ph->Ma * ph->Ma;  // types.jl:252
+  c->pc.vortex_Min = ph->vortex_Min;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete c;
+    return PDES_ERR_HIP;
+  }
+  c->own_stream = true;
+  const int dim = c->dim, Np = c->Np, Nf = c->Nf, nfaces = c->nfaces;
+
+  // host copies
+  c->h_w.assign(s->w, s->w + Np);
+  c->h_Q.assign(s->Q, s->Q + (size_t)dim * Np * Np);
+  c->h_interp.assign(s->interp, s->interp + (size_t)s->stencil * Nf);
+  c->h_perm.assign(s->perm, s->perm + (size_t)s->stencil * nfaces);
+  c->h_nbrperm.assign(s->nbrperm, s->nbrperm + (size_t)Nf * s->norient);
+  c->h_wface.assign(s->wface, s->wface + Nf);
+  c->h_jac.assign(m->jac, m->jac + (size_t)m->numEl * Np);
+  c->h_interfaces.assign(m->interfaces, m->interfaces + (size_t)5 * m->numInterfaces);
+  c->h_bndryfaces.assign(m->bndryfaces, m->bndryfaces + (size_t)2 * m->numBoundaryFaces);
+  c->h_bndry_offsets.assign(m->bndry_offsets, m->bndry_offsets + m->numBC + 1);
+  c->h_bc_type.assign(m->bc_type, m->bc_type + m->numBC);
+  c->h_nrm_bndry.assign(m->nrm_bndry, m->nrm_bndry + (size_t)m->numBoundaryFaces * Nf * dim);
+  c->h_coords_bndry.assign(m->coords_bndry, m->coords_bndry + (size_t)m->numBoundaryFaces * Nf * dim);
+
+  for (size_t x = 0; x < c->h_nbrperm.size(); ++x)
+    if (c->h_nbrperm[x] < 0 || c->h_nbrperm[x] >= Nf) {
+      set_error("nbrperm entry out of range");
+      pdes_ctx_destroy(c);
+      return PDES_ERR_ARG;
+    }
+  PDES_CHECK(s->stencil == (s->diag_e ? Nf : Np) && s->norient >= 1, PDES_ERR_ARG,
+             "stencil must equal Nf (diagonal E) or Np (dense face)");
+  // node -> stencil position per face
+  c->h_nodepos.assign((size_t)Np * nfaces, -1);
+  for (int st = 0; st < s->stencil; ++st)
+    for (int f = 0; f < nfaces; ++f) {
+      int node = s->perm[st * nfaces + f];
+      if (node < 0 || node >= Np) { set_error("perm entry out of range"); pdes_ctx_destroy(c); return PDES_ERR_ARG; }
+      c->h_nodepos[(size_t)node * nfaces + f] = st;
+    }
+
+  // element -> face connectivity
+  c->h_elface.assign((size_t)m->numEl * nfaces, ElFace{-2, -1, 0, 0});
+  for (int i = 0; i < m->numInterfaces; ++i) {
+    const int32_t* it = m->interfaces + 5 * i;
+    int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
+    if (eL < 0 || eL >= m->numEl || eR < 0 || eR >= m->numEl || fL < 0 || fL >= nfaces || fR < 0 ||
+        fR >= nfaces || o < 0 || o >= s->norient) {
+      set_error("interface %d out of range", i);
+      pdes_ctx_destroy(c);
+      return PDES_ERR_ARG;
+    }
+    c->h_elface[(size_t)eL * nfaces + fL] = ElFace{eR, i, fR | (o << 8), 0};
+    c->h_elface[(size_t)eR * nfaces + fR] = ElFace{eL, i, fL | (o << 8) | (1 << 16), 0};
+  }
+  for (int bc = 0; bc < m->numBC; ++bc)
+    for (int b = m->bndry_offsets[bc]; b < m->bndry_offsets[bc + 1]; ++b) {
+      int e = m->bndryfaces[2 * b], f = m->bndryfaces[2 * b + 1];
+      if (e < 0 || e >= m->numEl || f < 0 || f >= nfaces) {
+        set_error("boundary face %d out of range", b);
+        pdes_ctx_destroy(c);
+        return PDES_ERR_ARG;
+      }
+      c->h_elface[(size_t)e * nfaces + f] = ElFace{-1, b, 0, m->bc_type[bc]};
+    }
+  for (size_t x = 0; x < c->h_elface.size(); ++x)
+    if (c->h_elface[x].x == -2) {
+      set_error("element %zu face %zu is neither an interface nor a boundary face", x / nfaces, x % nfaces);
+      pdes_ctx_destroy(c);
+      return PDES_ERR_ARG;
+    }
+
+  // inverse mass matrix, calcMassMatrixInverse (src/Utils/mass_matrix.jl:20-55)
+  std::vector<double> Minv((size_t)c->numDof);
+  for (int e = 0; e < m->numEl; ++e)
+    for (int j = 0; j < Np; ++j) {
+      double v = 1.0 / (s->w[j] / m->jac[(size_t)e * Np + j]);
+      for (int k = 0; k < c->nd; ++k) Minv[((size_t)e * Np + j) * c->nd + k] = v;
+    }
+
+  int st = PDES_OK;
+  hipStream_t sm = c->stream;
+  const size_t nI = m->numInterfaces, nB = m->numBoundaryFaces;
+  if (!st) st = upload(&c->d_dxidx, m->dxidx, (size_t)m->numEl * Np * dim * dim, sm);
+  if (!st) st = upload(&c->d_elface, c->h_elface.data(), c->h_elface.size(), sm);
+  if (!st) st = upload(&c->d_nrm_face, m->nrm_face, nI * Nf * dim, sm);
+  if (!st) st = upload(&c->d_nrm_bndry, m->nrm_bndry, nB * Nf * dim, sm);
+  if (!st) st = upload(&c->d_coords_bndry, m->coords_bndry, nB * Nf * dim, sm);
+  if (!st) st = upload(&c->d_Q, s->Q, (size_t)dim * Np * Np, sm);
+  if (!st) st = upload(&c->d_interp, s->interp, (size_t)s->stencil * Nf, sm);
+  if (!st) st = upload(&c->d_perm, s->perm, (size_t)s->stencil * nfaces, sm);
+  if (!st) st = upload(&c->d_nbrperm, s->nbrperm, (size_t)Nf * s->norient, sm);
+  if (!st) st = upload(&c->d_nodepos, c->h_nodepos.data(), c->h_nodepos.size(), sm);
+  if (!st) st = upload(&c->d_wface, s->wface, (size_t)Nf, sm);
+  if (!st) st = upload(&c->d_Minv, Minv.data(), Minv.size(), sm);
+  if (!st) st = upload<double>(&c->d_q, nullptr, (size_t)c->numDof, sm);
+  if (!st) st = upload<double>(&c->d_res, nullptr, (size_t)c->numDof, sm);
+  if (!st) st = upload<double>(&c->d_work, nullptr, (size_t)c->numDof, sm);
+  if (!st) st = upload<double>(&c->d_work2, nullptr, (size_t)c->numDof, sm);
+  if (!st) st = upload<double>(&c->d_partial, nullptr, 4096, sm);
+  if (!st) st = upload<double>(&c->d_scalar, nullptr, 64, sm);
+  if (!st) st = upload<int32_t>(&c->d_flag, nullptr, 1, sm);
+  if (!st && hipMemsetAsync(c->d_flag, 0, sizeof(int32_t), sm) != hipSuccess) st = PDES_ERR_HIP;
+  if (!st && hipStreamSynchronize(sm) != hipSuccess) { set_error("upload failed"); st = PDES_ERR_HIP; }
+  if (st) {
+    pdes_ctx_destroy(c);
+    return st;
+  }
+  *out = c;
+  return PDES_OK;
+}
+
+int pdes_ctx_destroy(pdes_ctx* c) {
+  if (!c) return PDES_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  jacobian_free(c);
+  precond_free(c);
+  void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q,
+                  c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_q,
+                  c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  if (c->own_stream && c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return PDES_OK;
+}
+
+int pdes_ctx_numdof(const pdes_ctx* c, int64_t* numdof) {
+  PDES_CHECK(c && numdof, PDES_ERR_ARG, "null argument");
+  *numdof = c->numDof;
+  return PDES_OK;
+}
+
+int pdes_set_state(pdes_ctx* c, const double* q) {
+  PDES_CHECK(c && q, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(c->d_q, q, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_get_state(pdes_ctx* c, double* q) {
+  PDES_CHECK(c && q, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(q, c->d_q, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_get_residual(pdes_ctx* c, double* res) {
+  PDES_CHECK(c && res, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(res, c->d_res, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return sync_check_flag(c);
+}
+
+int pdes_state_ptr(pdes_ctx* c, double** q_dev, double** res_dev) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  if (q_dev) *q_dev = c->d_q;
+  if (res_dev) *res_dev = c->d_res;
+  return PDES_OK;
+}
+
+int pdes_sync(pdes_ctx* c) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return sync_check_flag(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,68 @@
+// ctx.hpp -- the library context (one per solve instance / HIP stream).
+#pragma once
+#include <vector>
+
+#include "residual_kernels.hpp"
+
+struct pdes_jacobian;
+struct pdes_precond;
+
+struct pdes_ctx {
+  // problem shape
+  int dim = 0, nd = 0, numEl = 0, Np = 0, Nf = 0, stencil = 0, norient = 0, nfaces = 0;
+  int diag_e = 0, numIface = 0, numBndry = 0, numBC = 0;
+  int64_t numDof = 0;
+  pdes_phys_desc phys{};
+  pdes::PhysConst pc{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  // device tables
+  double* d_dxidx = nullptr;
+  pdes::ElFace* d_elface = nullptr;
+  double* d_nrm_face = nullptr;
+  double* d_nrm_bndry = nullptr;
+  double* d_coords_bndry = nullptr;
+  double* d_Q = nullptr;
+  double* d_interp = nullptr;
+  int32_t* d_perm = nullptr;
+  int32_t* d_nbrperm = nullptr;
+  int32_t* d_nodepos = nullptr;
+  double* d_wface = nullptr;
+  double* d_Minv = nullptr;
+  // host copies used by assembly / functionals
+  std::vector<double> h_w, h_Q, h_interp, h_wface, h_jac, h_nrm_bndry, h_coords_bndry;
+  std::vector<int32_t> h_perm, h_nbrperm, h_nodepos, h_interfaces, h_bndryfaces, h_bndry_offsets, h_bc_type;
+  std::vector<pdes::ElFace> h_elface;
+  // state
+  double* d_q = nullptr;
+  double* d_res = nullptr;
+  double* d_work = nullptr;     // numDof scratch (RK4 / Jv direction)
+  double* d_work2 = nullptr;    // numDof scratch
+  double* d_partial = nullptr;  // reduction partials
+  double* d_scalar = nullptr;   // reduction results
+  int32_t* d_flag = nullptr;    // positivity flag
+  int partial_n = 0;
+  // launch configuration (elements per block), chosen per operator
+  int epb_weak = 0;
+  // solver state
+  pdes_jacobian* jac = nullptr;
+  pdes_precond* pc_obj = nullptr;
+
+  pdes::DevMesh devmesh() const {
+    return pdes::DevMesh{numEl, d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};
+  }
+  pdes::DevSBP devsbp() const {
+    return pdes::DevSBP{d_Q, d_interp, d_perm, d_nbrperm, d_nodepos, d_wface, norient};
+  }
+};
+
+namespace pdes {
+// residual launch (value run, or directional derivative when v != nullptr)
+int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
+                    bool check);
+// deterministic sum of x_i^2 * w_i (w may be null) into *out_dev
+int launch_wnorm2(pdes_ctx* c, const double* x, const double* w, int64_t n, double* out_dev);
+int sync_check_flag(pdes_ctx* c);
+int jacobian_free(pdes_ctx* c);
+int precond_free(pdes_ctx* c);
+}  // namespace pdes

@@ -1,0 +1,321 @@
+// euler_device.hpp -- Euler point functions on the device, templated on the
+// space dimension and on the scalar type (double, or dual<N> for Jacobian
+// blocks and Jacobian-vector products).  Each mirrors the reference function
+// cited beside it; the arithmetic sequence is kept so that results agree with
+// the reference to rounding.
+#pragma once
+#include "pdes_common.hpp"
+
+namespace pdes {
+
+struct PhysConst {
+  double gamma, gamma_1, R, cv;
+  double Ma, aoa, rho_free, E_free;
+  double vortex_Min;
+};
+
+// calcPressure  src/solver/euler/euler_funcs.jl:856-863 (2D), 897-903 (3D)
+template <int DIM, class T>
+__device__ __forceinline__ T pressure(const PhysConst& p, const T* q) {
+  if constexpr (DIM == 2) return p.gamma_1 * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+  else return p.gamma_1 * (q[4] - 0.5 * (q[1] * q[1] + q[2] * q[2] + q[3] * q[3]) / q[0]);
+}
+
+// calcEulerFlux  euler_funcs.jl:512-536 (2D), 749-774 (3D)
+template <int DIM, class T>
+__device__ __forceinline__ void euler_flux(const PhysConst& p, const T* q, const double* dir, T* F) {
+  T press = pressure<DIM>(p, q);
+  if constexpr (DIM == 2) {
+    T U = (q[1] * dir[0] + q[2] * dir[1]) / q[0];
+    F[0] = q[0] * U;
+    F[1] = q[1] * U + dir[0] * press;
+    F[2] = q[2] * U + dir[1] * press;
+    F[3] = (q[3] + press) * U;
+  } else {
+    T U = (q[1] * dir[0] + q[2] * dir[1] + q[3] * dir[2]) / q[0];
+    F[0] = q[0] * U;
+    F[1] = q[1] * U + dir[0] * press;
+    F[2] = q[2] * U + dir[1] * press;
+    F[3] = q[3] * U + dir[2] * press;
+    F[4] = (q[4] + press) * U;
+  }
+}
+
+// RoeSolver + calcSAT  src/solver/euler/bc_solvers.jl:29-103 / 207-310 (2D),
+// 111-187 / 313-420 (3D).  flux = SAT(qL - qR) + F_euler(qL).n
+template <int DIM, class T>
+__device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T* qg, const double* nrm, T* flux) {
+  constexpr int ND = DIM + 2;
+  const double gamma = p.gamma, gami = p.gamma_1;
+  const double sat_Vn = 0.025, sat_Vl = 0.025, tau = 1.0;
+  T u, v, w, H;
+  {
+    T fac = 1.0 / q[0];
+    T uL = q[1] * fac, vL = q[2] * fac;
+    T wL = 0.0;
+    T phi;
+    if constexpr (DIM == 2) phi = 0.5 * (uL * uL + vL * vL);
+    else { wL = q[3] * fac; phi = 0.5 * (uL * uL + vL * vL + wL * wL); }
+    T HL = gamma * q[ND - 1] * fac - gami * phi;
+    fac = 1.0 / qg[0];
+    T uR = qg[1] * fac, vR = qg[2] * fac;
+    T wR = 0.0;
+    if constexpr (DIM == 2) phi = 0.5 * (uR * uR + vR * vR);
+    else { wR = qg[3] * fac; phi = 0.5 * (uR * uR + vR * vR + wR * wR); }
+    T HR = gamma * qg[ND - 1] * fac - gami * phi;
+    T sqL = dsqrt(q[0]);
+    T sqR = dsqrt(qg[0]);
+    fac = 1.0 / (sqL + sqR);
+    u = (sqL * uL + sqR * uR) * fac;
+    v = (sqL * vL + sqR * vR) * fac;
+    if constexpr (DIM == 3) w = (sqL * wL + sqR * wR) * fac;
+    H = (sqL * HL + sqR * HR) * fac;
+  }
+  T dq[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) dq[i] = q[i] - qg[i];
+  // calcSAT
+  double nx = nrm[0], ny = nrm[1], nz = 0.0;
+  double dA;
+  T phi, Un;
+  if constexpr (DIM == 2) {
+    dA = sqrt(nx * nx + ny * ny);
+    Un = u * nx + v * ny;
+    phi = 0.5 * (u * u + v * v);
+  } else {
+    nz = nrm[2];
+    dA = sqrt(nx * nx + ny * ny + nz * nz);
+    phi = 0.5 * (u * u + v * v + w * w);
+    Un = u * nx + v * ny + w * nz;
+  }
+  T a = dsqrt(gami * (H - phi));
+  T lambda1 = Un + dA * a;
+  T lambda2 = Un - dA * a;
+  T lambda3 = Un;
+  T rhoA = absvalue(Un) + dA * a;
+  lambda1 = 0.5 * (tau * jmax(absvalue(lambda1), sat_Vn * rhoA) - lambda1);
+  lambda2 = 0.5 * (tau * jmax(absvalue(lambda2), sat_Vn * rhoA) - lambda2);
+  lambda3 = 0.5 * (tau * jmax(absvalue(lambda3), sat_Vl * rhoA) - lambda3);
+  T sat[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) sat[i] = lambda3 * dq[i];
+  T E1dq[ND], E2dq[ND];
+  T vel[3] = {u, v, w};
+  double n3[3] = {nx, ny, nz};
+  // E1 dq, E2 dq
+  {
+    T e1 = phi * dq[0] - u * dq[1] - v * dq[2];
+    if constexpr (DIM == 3) e1 = e1 - w * dq[3];
+    e1 = e1 + dq[ND - 1];
+    E1dq[0] = e1;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) E1dq[1 + d] = e1 * vel[d];
+    E1dq[ND - 1] = e1 * H;
+    T e2 = -Un * dq[0] + nx * dq[1] + ny * dq[2];
+    if constexpr (DIM == 3) e2 = e2 + nz * dq[3];
+    E2dq[0] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) E2dq[1 + d] = e2 * n3[d];
+    E2dq[ND - 1] = e2 * Un;
+    T tmp1 = 0.5 * (lambda1 + lambda2) - lambda3;
+    T tmp2 = gami / (a * a);
+    double tmp3 = 1.0 / (dA * dA);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) sat[i] = sat[i] + tmp1 * (tmp2 * E1dq[i] + tmp3 * E2dq[i]);
+    // E3 dq, E4 dq
+    E1dq[0] = e2;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) E1dq[1 + d] = e2 * vel[d];
+    E1dq[ND - 1] = e2 * H;
+    E2dq[0] = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) E2dq[1 + d] = e1 * n3[d];
+    E2dq[ND - 1] = e1 * Un;
+    tmp1 = 0.5 * (lambda1 - lambda2) / (dA * a);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) sat[i] = sat[i] + tmp1 * (E1dq[i] + gami * E2dq[i]);
+  }
+  T ef[ND];
+  euler_flux<DIM>(p, q, nrm, ef);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) flux[i] = sat[i] + ef[i];
+}
+
+// logavg  bc_solvers.jl:942-956
+template <class T>
+__device__ __forceinline__ T logavg(const T& aL, const T& aR) {
+  T xi = aL / aR;
+  T f = (xi - 1.0) / (xi + 1.0);
+  T u = f * f;
+  T F;
+  if (re(u) < 1e-3) F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
+  else F = (dlog(xi) / 2.0) / f;
+  return (aL + aR) / (2.0 * F);
+}
+
+// Ismail-Roe parameter vector of one state: z1 = sqrt(rho/p), z_{1+d} = z1 u_d,
+// z_last = sqrt(rho p)  (the per-state part of calcEulerFlux_IR, bc_solvers.jl:776-805)
+template <int DIM, class T>
+__device__ __forceinline__ void ir_zvars(const PhysConst& p, const T* q, T* z) {
+  T pr = pressure<DIM>(p, q);
+  T z1 = dsqrt(q[0] / pr);
+  z[0] = z1;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) z[1 + d] = z1 * q[1 + d] / q[0];
+  z[DIM + 1] = dsqrt(q[0] * pr);
+}
+
+// calcEulerFlux_IR, multi-direction  bc_solvers.jl:776-805 (2D), 842-874 (3D),
+// from precomputed z vectors.  dir: [x][ndir] ; F: [ndir][ndof]
+template <int DIM, int NDIR, class T>
+__device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const T* zR,
+                                          const double* dir, T* F) {
+  constexpr int ND = DIM + 2;
+  const double gamma = p.gamma, gamma_1 = p.gamma_1;
+  T la = logavg(zL[ND - 1], zR[ND - 1]);
+  T zs = zL[0] + zR[0];
+  T rho_hat = 0.5 * zs * la;
+  T vh[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) vh[d] = (zL[1 + d] + zR[1 + d]) / zs;
+  T p1_hat = (zL[ND - 1] + zR[ND - 1]) / zs;
+  T p2_hat = ((gamma + 1) / (2 * gamma)) * la / logavg(zL[0], zR[0]) +
+             (gamma_1 / (2 * gamma)) * (zL[ND - 1] + zR[ND - 1]) / zs;
+  T ke = vh[0] * vh[0] + vh[1] * vh[1];
+  if constexpr (DIM == 3) ke = ke + vh[2] * vh[2];
+  T h_hat = gamma * p2_hat / (rho_hat * gamma_1) + 0.5 * ke;
+#pragma unroll
+  for (int i = 0; i < NDIR; ++i) {
+    T un = dir[0 * NDIR + i] * vh[0] + dir[1 * NDIR + i] * vh[1];
+    if constexpr (DIM == 3) un = un + dir[2 * NDIR + i] * vh[2];
+    T mv_n = rho_hat * un;
+    F[i * ND + 0] = mv_n;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) F[i * ND + 1 + d] = mv_n * vh[d] + dir[d * NDIR + i] * p1_hat;
+    F[i * ND + ND - 1] = mv_n * h_hat;
+  }
+}
+
+// convertToIR_  src/solver/euler/conversion.jl:160-205
+template <int DIM, class T>
+__device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* qe) {
+  constexpr int ND = DIM + 2;
+  const double gamma = p.gamma, gamma_1 = p.gamma_1, gamma_1i = 1.0 / gamma_1;
+  T k1 = qc[1] * qc[1] + qc[2] * qc[2];
+  if constexpr (DIM == 3) k1 = k1 + qc[3] * qc[3];
+  k1 = 0.5 * k1 / qc[0];
+  T rho_int = qc[ND - 1] - k1;
+  T s = dlog(gamma_1 * rho_int / dpow(qc[0], gamma));
+  T fac = 1.0 / rho_int;
+  T tmp1 = -qc[0] * fac * gamma_1i;
+  qe[0] = ((rho_int * (gamma + 1.0 - s) - qc[ND - 1]) * fac) * gamma_1i;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) qe[1 + d] = qc[1 + d] * fac * gamma_1i;
+  qe[ND - 1] = tmp1;
+}
+
+// IRSLF face flux: calcEulerFlux_IRSLF bc_solvers.jl:898-909 ->
+// applyEntropyKernel_diagE faceElementIntegrals.jl:510-575 -> LFKernel :455-470
+// with getIRA0 IR_stab.jl:15-110 and getLambdaMax euler_funcs.jl:1887-1910.
+template <int DIM, class T>
+__device__ __forceinline__ void irslf_flux(const PhysConst& p, const T* qL, const T* qR,
+                                           const double* nrm, T* F) {
+  constexpr int ND = DIM + 2;
+  T zL[ND], zR[ND];
+  ir_zvars<DIM>(p, qL, zL);
+  ir_zvars<DIM>(p, qR, zR);
+  ir_flux_z<DIM, 1>(p, zL, zR, nrm, F);
+  T qa[ND], vL[ND], vR[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) qa[i] = 0.5 * (qL[i] + qR[i]);
+  to_ir_vars<DIM>(p, qL, vL);
+  to_ir_vars<DIM>(p, qR, vR);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) vL[i] = vL[i] - vR[i];
+  // A0 = dq/dv at q_avg (getIRA0), lambda_max at q_avg
+  const double gamma = p.gamma, gamma_1 = p.gamma_1;
+  T pr = pressure<DIM>(p, qa);
+  T rho = qa[0];
+  T rhoinv = 1.0 / rho;
+  T h = (qa[ND - 1] + pr) * rhoinv;
+  T a2 = gamma * pr * rhoinv;
+  T A0[ND][ND];  // A0[row][col]
+  A0[0][0] = rho;
+#pragma unroll
+  for (int r = 1; r < ND; ++r) { A0[r][0] = qa[r]; A0[0][r] = qa[r]; }
+#pragma unroll
+  for (int r = 1; r <= DIM; ++r)
+#pragma unroll
+    for (int c = 1; c <= DIM; ++c) {
+      if (r == c) A0[r][c] = qa[r] * qa[r] * rhoinv + pr;
+      else if (c == 1 && r == 2) A0[r][c] = qa[1] * qa[2] * rhoinv;     // A0[3,2] 2D: rhou*rhov*rhoinv
+      else if (r == 1 && c == 2) A0[r][c] = qa[1] * qa[2] / rho;        // A0[2,3]: rhou*rhov/rho
+      else if (c == 1) A0[r][c] = qa[1] * qa[r] * rhoinv;               // rhou*rhow*rhoinv
+      else if (r == 1) A0[r][c] = qa[c] * qa[1] * rhoinv;               // rhow*rhou*rhoinv
+      else if (c == 2) A0[r][c] = qa[2] * qa[r] * rhoinv;               // rhov*rhow*rhoinv
+      else A0[r][c] = qa[c] * qa[r] * rhoinv;                           // rhow*rhov*rhoinv
+    }
+#pragma unroll
+  for (int d = 1; d <= DIM; ++d) { A0[ND - 1][d] = qa[d] * h; A0[d][ND - 1] = h * qa[d]; }
+  A0[ND - 1][ND - 1] = rho * h * h - a2 * pr / gamma_1;
+  // getLambdaMax(params, q_avg, dir)
+  T Un = 0.0;
+  double dA = 0.0;
+  T aL = dsqrt(gamma * pr * rhoinv);
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    Un = Un + nrm[i] * qa[i + 1] * rhoinv;
+    dA += nrm[i] * nrm[i];
+  }
+  dA = sqrt(dA);
+  T lambda_max = absvalue3(Un) + dA * aL;
+#pragma unroll
+  for (int r = 0; r < ND; ++r) {
+    T acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < ND; ++c) acc = acc + A0[r][c] * vL[c];
+    F[r] = F[r] + acc * lambda_max;
+  }
+}
+
+// calcIsentropicVortex  src/solver/euler/common_funcs.jl:25-76 (inflow Mach a parameter)
+__device__ __forceinline__ void isentropic_vortex(const PhysConst& p, const double* coords, double* sol) {
+  double x = coords[0], y = coords[1];
+  double gamma = p.gamma;
+  const double r_in = 1, rho_in = 2;
+  double M_in = p.vortex_Min, p_in = 1 / gamma;
+  double r = sqrt(x * x + y * y);
+  double theta = atan2(y, x);
+  double tmp1 = ((gamma - 1) / 2) * M_in * M_in;
+  double rho_r = rho_in * pow(1 + tmp1 * (1 - (r_in * r_in) / (r * r)), 1 / (gamma - 1));
+  double p_r = p_in * pow(rho_r / rho_in, gamma);
+  double a_r = sqrt(gamma * p_r / rho_r);
+  double M_r = sqrt((2 / (gamma - 1)) * pow(rho_in / rho_r, gamma - 1) * (1 + tmp1) - 2 / (gamma - 1));
+  double U_r = M_r * a_r;
+  double u_r = U_r * sin(theta);
+  double v_r = -U_r * cos(theta);
+  double e_r = p.cv * p_r / (rho_r * p.R);
+  double E_r = rho_r * e_r + 0.5 * rho_r * U_r * U_r;
+  sol[0] = rho_r;
+  sol[1] = rho_r * u_r;
+  sol[2] = rho_r * v_r;
+  sol[3] = E_r;
+}
+
+// calcFreeStream  common_funcs.jl:312-347
+template <int DIM>
+__device__ __forceinline__ void free_stream(const PhysConst& p, double* sol) {
+  double rho = p.rho_free;
+  sol[0] = rho;
+  sol[DIM + 1] = p.E_free;
+  if constexpr (DIM == 2) {
+    sol[1] = rho * p.Ma * cos(p.aoa);
+    sol[2] = rho * p.Ma * sin(p.aoa);
+  } else {
+    sol[1] = rho * p.Ma * cos(p.aoa);
+    sol[2] = 0.0;
+    sol[3] = -rho * p.Ma * sin(p.aoa);
+  }
+}
+
+}  // namespace pdes

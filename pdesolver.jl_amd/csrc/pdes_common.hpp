@@ -1,0 +1,147 @@
+// pdes_common.hpp -- shared host/device helpers of the gfx950 library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/pdes.h"
+
+namespace pdes {
+
+void set_error(const char* fmt, ...);
+
+#define PDES_HIP(call)                                                          \
+  do {                                                                          \
+    hipError_t _e = (call);                                                     \
+    if (_e != hipSuccess) {                                                     \
+      ::pdes::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call,              \
+                        hipGetErrorString(_e));                                 \
+      return PDES_ERR_HIP;                                                      \
+    }                                                                           \
+  } while (0)
+
+#define PDES_CHECK(cond, code, ...)                                             \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      ::pdes::set_error(__VA_ARGS__);                                           \
+      return (code);                                                            \
+    }                                                                           \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers.  With h -> 0 the complex step of the reference
+// (q + i h v, src/interface2.jl:454-498) propagates exactly these first-order
+// derivatives; every branch of the flux functions tests the real part, as
+// the reference's complex-safe helpers do (src/Utils/complexify.jl).
+// ---------------------------------------------------------------------------
+template <int N>
+struct dual {
+  double v;
+  double d[N];
+  __host__ __device__ dual() {}
+  __host__ __device__ dual(double x) : v(x) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) d[i] = 0.0;
+  }
+};
+
+template <int N> __device__ __forceinline__ double re(const dual<N>& a) { return a.v; }
+__device__ __forceinline__ double re(double a) { return a; }
+
+#define DUAL_BINOP(OP, VEXPR, DEXPR)                                              \
+  template <int N> __device__ __forceinline__ dual<N> operator OP(const dual<N>& a, const dual<N>& b) { \
+    dual<N> r; r.v = VEXPR;                                                       \
+    _Pragma("unroll") for (int i = 0; i < N; ++i) r.d[i] = DEXPR;                 \
+    return r; }
+DUAL_BINOP(+, a.v + b.v, a.d[i] + b.d[i])
+DUAL_BINOP(-, a.v - b.v, a.d[i] - b.d[i])
+DUAL_BINOP(*, a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+#undef DUAL_BINOP
+
+template <int N> __device__ __forceinline__ dual<N> operator/(const dual<N>& a, const dual<N>& b) {
+  dual<N> r;
+  double inv = 1.0 / b.v;
+  r.v = a.v * inv;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * inv;
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N> operator-(const dual<N>& a) {
+  dual<N> r; r.v = -a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+// mixed dual/double
+template <int N> __device__ __forceinline__ dual<N> operator+(const dual<N>& a, double b) { dual<N> r = a; r.v += b; return r; }
+template <int N> __device__ __forceinline__ dual<N> operator+(double b, const dual<N>& a) { dual<N> r = a; r.v += b; return r; }
+template <int N> __device__ __forceinline__ dual<N> operator-(const dual<N>& a, double b) { dual<N> r = a; r.v -= b; return r; }
+template <int N> __device__ __forceinline__ dual<N> operator-(double b, const dual<N>& a) {
+  dual<N> r; r.v = b - a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -a.d[i];
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N> operator*(const dual<N>& a, double b) {
+  dual<N> r; r.v = a.v * b;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * b;
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N> operator*(double b, const dual<N>& a) { return a * b; }
+template <int N> __device__ __forceinline__ dual<N> operator/(const dual<N>& a, double b) {
+  dual<N> r; double inv = 1.0 / b; r.v = a.v / b;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * inv;
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N> operator/(double a, const dual<N>& b) {
+  dual<N> r; double inv = 1.0 / b.v; r.v = a * inv;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = -r.v * b.d[i] * inv;
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N>& operator+=(dual<N>& a, const dual<N>& b) { a = a + b; return a; }
+template <int N> __device__ __forceinline__ dual<N>& operator-=(dual<N>& a, const dual<N>& b) { a = a - b; return a; }
+
+template <int N> __device__ __forceinline__ dual<N> dsqrt(const dual<N>& a) {
+  dual<N> r; r.v = sqrt(a.v); double f = 0.5 / r.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N> dlog(const dual<N>& a) {
+  dual<N> r; r.v = log(a.v); double f = 1.0 / a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
+  return r;
+}
+template <int N> __device__ __forceinline__ dual<N> dpow(const dual<N>& a, double g) {
+  dual<N> r; r.v = pow(a.v, g); double f = g * r.v / a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
+  return r;
+}
+__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
+__device__ __forceinline__ double dlog(double a) { return log(a); }
+__device__ __forceinline__ double dpow(double a, double g) { return pow(a, g); }
+
+// absvalue (src/Utils/complexify.jl:25-45): sign flip by the real part
+__device__ __forceinline__ double absvalue(double x) { return fabs(x); }
+template <int N> __device__ __forceinline__ dual<N> absvalue(const dual<N>& x) { return x.v < 0 ? -x : x; }
+// Julia max(x, y) = isless(y, x) ? x : y, ordered by the real part
+template <class T> __device__ __forceinline__ T jmax(const T& x, const T& y) { return re(y) < re(x) ? x : y; }
+// absvalue3 (src/Utils/complexify.jl:161-172)
+template <class T> __device__ __forceinline__ T absvalue3(const T& val) {
+  const double delta = 1e-7;
+  T val1 = absvalue(val);
+  if (re(val1) > delta) return val1;
+  return ((val * val) / delta + delta) / 2.0;
+}
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace pdes

@@ -1,0 +1,243 @@
+// residual.hip -- operator-shape dispatch of the fused residual kernel and the
+// reduction / vector kernels used around it.
+#include "ctx.hpp"
+
+namespace pdes {
+
+// Operator shapes compiled for gfx950: triangle Omega p=1..4, triangle
+// diagonal-E p=1..4, tetrahedron Omega p=1..2.
+#define PDES_FOR_EACH_SHAPE(X)          \
+  X(2, 3, 2, 3, false)                  \
+  X(2, 6, 3, 6, false)                  \
+  X(2, 10, 4, 10, false)                \
+  X(2, 15, 5, 15, false)                \
+  X(2, 7, 3, 3, true)                   \
+  X(2, 12, 4, 4, true)                  \
+  X(2, 18, 5, 5, true)                  \
+  X(2, 27, 6, 6, true)                  \
+  X(3, 4, 3, 4, false)                  \
+  X(3, 10, 6, 10, false)
+
+template <class Op, class T>
+constexpr int epb_for() {
+  // elements per block: fill up to 256 threads (128 for derivative runs)
+  constexpr int maxthreads = std::is_same<T, double>::value ? 256 : 128;
+  return maxthreads / Op::NP;
+}
+
+template <class Op, int VOL, int FLUX, class T>
+static int launch_one(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
+                      int32_t* flag) {
+  constexpr int EPB = epb_for<Op, T>();
+  const int threads = EPB * Op::NP;
+  const int blocks = (int)ceil_div(c->numEl, EPB);
+  hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T>), dim3(blocks), dim3(threads), 0,
+                     c->stream, c->devmesh(), c->devsbp(), c->pc, q, v, res, jv, flag);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+template <class Op, class T>
+static int launch_shape(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
+                        int32_t* flag) {
+  const int vol = c->phys.volume_integral_type, flux = c->phys.face_flux;
+  if (vol == 1 && flux == PDES_FLUX_ROE) return launch_one<Op, VOL_WEAK, PDES_FLUX_ROE, T>(c, q, v, res, jv, flag);
+  if constexpr (Op::DIAGE && Op::DIM == 2) {
+    if (vol == 2 && flux == PDES_FLUX_IRSLF)
+      return launch_one<Op, VOL_SPLIT, PDES_FLUX_IRSLF, T>(c, q, v, res, jv, flag);
+  }
+  set_error("unsupported combination volume_integral_type=%d face_flux=%d for this operator", vol, flux);
+  return PDES_ERR_UNSUPPORTED;
+}
+
+int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, bool check) {
+  int32_t* flag = check ? c->d_flag : nullptr;
+#define PDES_DISPATCH(D, NP, NF, ST, DE)                                                        \
+  if (c->dim == D && c->Np == NP && c->Nf == NF && c->stencil == ST && (c->diag_e != 0) == DE) { \
+    using Op = OpShape<D, NP, NF, ST, DE>;                                                      \
+    if (v) return launch_shape<Op, dual<1>>(c, q, v, res, jv, flag);                            \
+    return launch_shape<Op, double>(c, q, v, res, jv, flag);                                    \
+  }
+  PDES_FOR_EACH_SHAPE(PDES_DISPATCH)
+#undef PDES_DISPATCH
+  set_error("no compiled kernel for dim=%d Np=%d Nf=%d stencil=%d diag_e=%d", c->dim, c->Np, c->Nf,
+            c->stencil, c->diag_e);
+  return PDES_ERR_UNSUPPORTED;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic weighted sum of squares: fixed per-block partial order, then
+// one block sums the partials in index order.
+// ---------------------------------------------------------------------------
+constexpr int RED_THREADS = 256;
+constexpr int RED_BLOCKS = 1024;
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+__global__ void wnorm2_partial(const double* __restrict__ x, const double* __restrict__ w, int64_t n,
+                               double* __restrict__ partial) {
+  __shared__ double sh[RED_THREADS];
+  double acc = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double xi = x[i];
+    acc += w ? xi * xi * w[i] : xi * xi;
+  }
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
+  __shared__ double sh[RED_THREADS];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+int launch_wnorm2(pdes_ctx* c, const double* x, const double* w, int64_t n, double* out_dev) {
+  int blocks = (int)std::min<int64_t>(RED_BLOCKS, std::max<int64_t>(1, ceil_div(n, RED_THREADS)));
+  hipLaunchKernelGGL(wnorm2_partial, dim3(blocks), dim3(RED_THREADS), 0, c->stream, x, w, n, c->d_partial);
+  hipLaunchKernelGGL(sum_partials, dim3(1), dim3(RED_THREADS), 0, c->stream, c->d_partial, blocks, out_dev);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+// RK4 stage update (rk4.jl:238-319 with pde_post_func :446-460, Minv scaling):
+//   k = Minv .* res ; acc += wacc * k ; q_next = q_old + cstage * k
+__global__ void rk4_stage_update(const double* __restrict__ res, const double* __restrict__ Minv,
+                                 const double* __restrict__ q_old, double* __restrict__ acc,
+                                 double* __restrict__ q_next, double wacc, double cstage, int first,
+                                 int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double k = res[i] * Minv[i];
+    double a = first ? wacc * k : acc[i] + wacc * k;
+    acc[i] = a;
+    q_next[i] = q_old[i] + cstage * k;
+  }
+}
+
+__global__ void rk4_finish(double* __restrict__ q_old, const double* __restrict__ acc, double h6, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    q_old[i] = q_old[i] + h6 * acc[i];
+}
+
+}  // namespace pdes
+
+using namespace pdes;
+
+extern "C" {
+
+int pdes_eval_residual(pdes_ctx* c, double t) {
+  (void)t;
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  return launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+}
+
+int pdes_residual_norm(pdes_ctx* c, double* norm) {
+  PDES_CHECK(c && norm, PDES_ERR_ARG, "null argument");
+  int st = launch_wnorm2(c, c->d_res, c->d_Minv, c->numDof, c->d_scalar);
+  if (st) return st;
+  double v = 0;
+  PDES_HIP(hipMemcpyAsync(&v, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  st = sync_check_flag(c);
+  if (st) return st;
+  *norm = std::sqrt(v);
+  return PDES_OK;
+}
+
+int pdes_jv_complex_step(pdes_ctx* c, const double* v_host, double* b_host) {
+  PDES_CHECK(c && v_host && b_host, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(c->d_work, v_host, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = launch_residual(c, c->d_q, c->d_work, nullptr, c->d_work2, false);
+  if (st) return st;
+  PDES_HIP(hipMemcpyAsync(b_host, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+// Device-resident Jv: b = dR/dq v for device vectors (matrix-free linear operator).
+int pdes_jv_device(pdes_ctx* c, const double* v_dev, double* b_dev) {
+  PDES_CHECK(c && v_dev && b_dev, PDES_ERR_ARG, "null argument");
+  return launch_residual(c, c->d_q, v_dev, nullptr, b_dev, false);
+}
+
+// Times `iters` back-to-back residual evaluations on the context stream with
+// HIP events; *avg_ms is the average per evaluation (one kernel launch each).
+int pdes_time_residual(pdes_ctx* c, int iters, double* avg_ms) {
+  PDES_CHECK(c && avg_ms && iters > 0, PDES_ERR_ARG, "bad argument");
+  hipEvent_t a, b;
+  PDES_HIP(hipEventCreate(&a));
+  PDES_HIP(hipEventCreate(&b));
+  PDES_HIP(hipEventRecord(a, c->stream));
+  for (int it = 0; it < iters; ++it) {
+    int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false);
+    if (st) return st;
+  }
+  PDES_HIP(hipEventRecord(b, c->stream));
+  PDES_HIP(hipEventSynchronize(b));
+  float ms = 0;
+  PDES_HIP(hipEventElapsedTime(&ms, a, b));
+  PDES_HIP(hipEventDestroy(a));
+  PDES_HIP(hipEventDestroy(b));
+  *avg_ms = ms / iters;
+  return PDES_OK;
+}
+
+// rk4 (src/NonlinearSolvers/rk4.jl:144-344): nsteps classical RK4 steps of
+// dq/dt = Minv .* R(q) on the device state; 4 residual evaluations per step.
+// If kernel_ms is not null, the residual launches of the whole run are
+// bracketed by events: *elapsed_ms gets the wall time of all steps.
+int pdes_rk4(pdes_ctx* c, int nsteps, double h, double* elapsed_ms) {
+  PDES_CHECK(c && nsteps >= 0, PDES_ERR_ARG, "bad argument");
+  const int64_t n = c->numDof;
+  const int threads = 256;
+  const int blocks = (int)std::min<int64_t>(8192, ceil_div(n, threads));
+  double* q_old = c->d_q;
+  double* acc = c->d_work;
+  double* q_stage = c->d_work2;
+  hipEvent_t a = nullptr, b = nullptr;
+  if (elapsed_ms) {
+    PDES_HIP(hipEventCreate(&a));
+    PDES_HIP(hipEventCreate(&b));
+    PDES_HIP(hipEventRecord(a, c->stream));
+  }
+  const double cst[4] = {h / 2, h / 2, h, 0.0};
+  const double wst[4] = {1.0, 2.0, 2.0, 1.0};
+  for (int s = 0; s < nsteps; ++s) {
+    for (int stage = 0; stage < 4; ++stage) {
+      const double* qin = stage == 0 ? q_old : q_stage;
+      int st = launch_residual(c, qin, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+      if (st) return st;
+      hipLaunchKernelGGL(rk4_stage_update, dim3(blocks), dim3(threads), 0, c->stream, c->d_res, c->d_Minv,
+                         q_old, acc, q_stage, wst[stage], cst[stage], stage == 0 ? 1 : 0, n);
+    }
+    hipLaunchKernelGGL(rk4_finish, dim3(blocks), dim3(threads), 0, c->stream, q_old, acc, h / 6, n);
+  }
+  PDES_HIP(hipGetLastError());
+  if (elapsed_ms) {
+    PDES_HIP(hipEventRecord(b, c->stream));
+    PDES_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    PDES_HIP(hipEventElapsedTime(&ms, a, b));
+    *elapsed_ms = ms;
+    PDES_HIP(hipEventDestroy(a));
+    PDES_HIP(hipEventDestroy(b));
+  }
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return sync_check_flag(c);
+}
+
+}  // extern "C"

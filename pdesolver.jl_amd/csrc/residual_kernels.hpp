@@ -1,0 +1,353 @@
+// residual_kernels.hpp -- the fused DG-SBP Euler residual kernel for gfx950.
+//
+// One launch evaluates the whole of evalResidual (src/solver/euler/euler.jl:
+// 111-175) for one operator:
+//   * volume term: weak form res += sum_d Q_d^T F_d (euler.jl:628-658), or the
+//     entropy-stable split form res_j -= sum_k 2 S_jk F*(q_j, q_k) with the
+//     Ismail-Roe flux (euler_funcs.jl:240-288);
+//   * face term: R q on both sides, numerical flux, res -= R^T B F
+//     (flux.jl:37-125, euler.jl:770-802);
+//   * boundary term: R q, BC state, Roe flux, res -= R^T B F (bc.jl:49-328).
+//
+// Layout: one thread per (element, volume node); a block holds EPB whole
+// elements.  The element state, volume fluxes and face contributions are
+// staged in LDS.  Faces are evaluated by the element that owns the residual
+// rows ("owner pull"): each element recomputes the flux of each of its faces
+// with the elementL orientation, so both neighbours obtain bit-identical
+// +/- contributions without atomics and without a second pass over HBM.
+#pragma once
+#include "euler_device.hpp"
+
+namespace pdes {
+
+// element -> face connectivity record
+//   x: neighbour element (>= 0) or -1 for a boundary face
+//   y: interface index (interior) or boundary face index
+//   z: neighbour face | orient << 8 | side << 16  (side 0: this element is elementL)
+//   w: BC functor of a boundary face (PDES_BC_*)
+struct ElFace { int32_t x, y, z, w; };
+
+struct DevMesh {
+  int32_t numEl;
+  const double* __restrict__ dxidx;       // [e][j][x][xi]
+  const ElFace* __restrict__ elface;      // [e][f]
+  const double* __restrict__ nrm_face;    // [iface][k][dim]
+  const double* __restrict__ nrm_bndry;   // [b][k][dim]
+  const double* __restrict__ coords_bndry;// [b][k][dim]
+};
+
+struct DevSBP {
+  const double* __restrict__ Q;        // [d][i][j]
+  const double* __restrict__ interp;   // [s][k]
+  const int32_t* __restrict__ perm;    // [s][f]
+  const int32_t* __restrict__ nbrperm; // [k][o]
+  const int32_t* __restrict__ nodepos; // [i][f]: stencil position of node i on face f, -1 if none
+  const double* __restrict__ wface;    // [k]
+  int32_t norient;
+};
+
+template <int DIM_, int NP_, int NF_, int ST_, bool DIAGE_>
+struct OpShape {
+  static constexpr int DIM = DIM_, NP = NP_, NF = NF_, ST = ST_;
+  static constexpr bool DIAGE = DIAGE_;
+  static constexpr int ND = DIM_ + 2, NFACES = DIM_ + 1;
+};
+
+// scalar loads/stores for value (double) and directional-derivative (dual<1>) runs
+template <class T> struct Io;
+template <> struct Io<double> {
+  static __device__ __forceinline__ double load(const double* q, const double*, int64_t i) { return q[i]; }
+  static __device__ __forceinline__ void store(double* r, double*, int64_t i, double v) { r[i] = v; }
+};
+template <> struct Io<dual<1>> {
+  static __device__ __forceinline__ dual<1> load(const double* q, const double* v, int64_t i) {
+    dual<1> a; a.v = q[i]; a.d[0] = v[i]; return a;
+  }
+  static __device__ __forceinline__ void store(double* r, double* jv, int64_t i, const dual<1>& a) {
+    if (r) r[i] = a.v;
+    jv[i] = a.d[0];
+  }
+};
+
+enum { VOL_WEAK = 1, VOL_SPLIT = 2 };
+
+template <class Op, int VOL, int FLUX, int EPB, class T>
+struct ResidualSmem {
+  static constexpr int NP = Op::NP, ND = Op::ND, DIM = Op::DIM, NF = Op::NF, NFACES = Op::NFACES;
+  double Q[DIM][NP][NP];
+  double interp[Op::ST][NF];
+  int32_t perm[Op::ST][NFACES];
+  int32_t nodepos[NP][NFACES];
+  double wface[NF];
+  T q[EPB][NP][ND];
+  // weak form: volume fluxes; split form: IR z-variables and partner contributions
+  T F[EPB][(VOL == VOL_WEAK) ? DIM * NP * ND : NP * ND];
+  T C[EPB][(VOL == VOL_SPLIT) ? NP * ND : 1];
+  T G[EPB][NFACES][NF][ND];
+};
+
+template <class Op, int VOL, int FLUX, int EPB, class T>
+__global__ void __launch_bounds__(256)
+euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
+                      const double* __restrict__ qin, const double* __restrict__ vin,
+                      double* __restrict__ res, double* __restrict__ jv, int32_t* __restrict__ flag) {
+  constexpr int DIM = Op::DIM, NP = Op::NP, NF = Op::NF, ST = Op::ST, ND = Op::ND, NFACES = Op::NFACES;
+  using Smem = ResidualSmem<Op, VOL, FLUX, EPB, T>;
+  __shared__ Smem sm;
+
+  const int tid = threadIdx.x;
+  // stage operator tables
+  for (int x = tid; x < DIM * NP * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.Q[x];
+  for (int x = tid; x < ST * NF; x += blockDim.x) (&sm.interp[0][0])[x] = sbp.interp[x];
+  for (int x = tid; x < ST * NFACES; x += blockDim.x) (&sm.perm[0][0])[x] = sbp.perm[x];
+  for (int x = tid; x < NP * NFACES; x += blockDim.x) (&sm.nodepos[0][0])[x] = sbp.nodepos[x];
+  for (int x = tid; x < NF; x += blockDim.x) sm.wface[x] = sbp.wface[x];
+
+  const int el = tid / NP;
+  const int i = tid - el * NP;
+  const int64_t e = (int64_t)blockIdx.x * EPB + el;
+  const bool active = (el < EPB) && (e < mesh.numEl);
+
+  T qi[ND];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+      qi[k] = Io<T>::load(qin, vin, (e * NP + i) * ND + k);
+      sm.q[el][i][k] = qi[k];
+    }
+    // checkDensity / checkPressure (euler.jl:543-611)
+    if (flag) {
+      if (!(re(qi[0]) > 0.0) || !(re(pressure<DIM>(phys, qi)) > 0.0)) flag[0] = 1;
+    }
+  }
+  double dxi[DIM][DIM];  // dxidx[e][i][x][xi]
+  if (active) {
+#pragma unroll
+    for (int x = 0; x < DIM; ++x)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) dxi[x][d] = mesh.dxidx[((e * NP + i) * DIM + x) * DIM + d];
+  }
+
+  T ri[ND];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) ri[k] = 0.0;
+
+  if constexpr (VOL == VOL_WEAK) {
+    // getEulerFlux: flux in parametric direction d uses nrm = dxidx[d, :, j, e]
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+        double nrm[DIM];
+#pragma unroll
+        for (int x = 0; x < DIM; ++x) nrm[x] = dxi[x][d];
+        T Fd[ND];
+        euler_flux<DIM>(phys, qi, nrm, Fd);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) sm.F[el][(d * NP + i) * ND + k] = Fd[k];
+      }
+    }
+    __syncthreads();
+    // weakdifferentiate!(trans=true): res_i += sum_d sum_j Q_d[j][i] F_d[j]
+    if (active) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+#pragma unroll 4
+        for (int j = 0; j < NP; ++j) {
+          double qji = sm.Q[d][j][i];
+#pragma unroll
+          for (int k = 0; k < ND; ++k) ri[k] = ri[k] + qji * sm.F[el][(d * NP + j) * ND + k];
+        }
+    }
+  } else {
+    // split form (calcVolumeIntegralsSplitFormLinear): per-node IR z-variables in LDS,
+    // each unordered node pair evaluated once by the node that starts its circulant distance.
+    if (active) {
+      T z[ND];
+      ir_zvars<DIM>(phys, qi, z);
+#pragma unroll
+      for (int k = 0; k < ND; ++k) sm.F[el][i * ND + k] = z[k];
+    }
+    __syncthreads();
+    // circulant distance dd: node i evaluates the pair (i, i+dd mod NP); the
+    // partner's share goes through one LDS slot per node, one round per dd.
+    constexpr int NPAIR = NP / 2;
+    T zi[ND];
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < ND; ++k) zi[k] = sm.F[el][i * ND + k];
+    }
+    for (int dd = 1; dd <= NPAIR; ++dd) {
+      const bool half = (2 * dd == NP);            // even NP: distance NP/2 owned by i < NP/2
+      if (active) {
+        T cother[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) cother[k] = 0.0;
+        if (!(half && i >= NP / 2)) {
+          int kk = i + dd;
+          if (kk >= NP) kk -= NP;
+          T zk[ND];
+#pragma unroll
+          for (int k = 0; k < ND; ++k) zk[k] = sm.F[el][kk * ND + k];
+          // reference orientation: F(q_j, q_k) with j > k, direction rows of dxidx at node j
+          const bool i_is_j = i > kk;
+          const int jnode = i_is_j ? i : kk;
+          double nrmD[DIM * DIM];   // [x][d]
+#pragma unroll
+          for (int x = 0; x < DIM; ++x)
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = mesh.dxidx[((e * NP + jnode) * DIM + x) * DIM + d];
+          T Fd[DIM * ND];
+          if (i_is_j) ir_flux_z<DIM, DIM>(phys, zi, zk, nrmD, Fd);
+          else ir_flux_z<DIM, DIM>(phys, zk, zi, nrmD, Fd);
+          // res_a -= 2 S[a][b] F for both members a of the pair
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) {
+            const double s_ik = 0.5 * (sm.Q[d][i][kk] - sm.Q[d][kk][i]);
+#pragma unroll
+            for (int k = 0; k < ND; ++k) {
+              ri[k] = ri[k] - 2 * s_ik * Fd[d * ND + k];
+              cother[k] = cother[k] + 2 * s_ik * Fd[d * ND + k];   // -2 S[kk][i] F
+            }
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < ND; ++k) sm.C[el][i * ND + k] = cother[k];
+      }
+      __syncthreads();
+      if (active) {
+        int src = i - dd;
+        if (src < 0) src += NP;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) ri[k] = ri[k] + sm.C[el][src * ND + k];
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- faces: slot (f, k) with k in elementL's face-node numbering
+  if (active) {
+    for (int slot = i; slot < NFACES * NF; slot += NP) {
+      const int f = slot / NF;
+      const int k = slot - f * NF;
+      const ElFace rec = mesh.elface[e * NFACES + f];
+      T qL[ND], qR[ND], Fk[ND];
+      int m;
+      double sign;
+      if (rec.x < 0) {
+        // boundary: q_b = R q, state from the BC functor, RoeSolver
+        m = k;
+        sign = -1.0;
+        if constexpr (Op::DIAGE) {
+          const int node = sm.perm[k][f];
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qL[c] = sm.q[el][node][c];
+        } else {
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qL[c] = 0.0;
+          for (int s = 0; s < ST; ++s) {
+            double r = sm.interp[s][k];
+            const int node = sm.perm[s][f];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) qL[c] = qL[c] + r * sm.q[el][node][c];
+          }
+        }
+        const int64_t b = rec.y;
+        double qg[ND];
+        if constexpr (DIM == 2) {
+          if (rec.w == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(phys, mesh.coords_bndry + (b * NF + k) * DIM, qg);
+          else free_stream<DIM>(phys, qg);
+        } else {
+          free_stream<DIM>(phys, qg);
+        }
+#pragma unroll
+        for (int c = 0; c < ND; ++c) qR[c] = qg[c];
+        roe_flux<DIM>(phys, qL, qR, mesh.nrm_bndry + (b * NF + k) * DIM, Fk);
+      } else {
+        const int64_t nb = rec.x;
+        const int fN = rec.z & 0xff;
+        const int o = (rec.z >> 8) & 0xff;
+        const int side = (rec.z >> 16) & 1;
+        const int kR = sbp.nbrperm[k * sbp.norient + o];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) { qL[c] = 0.0; qR[c] = 0.0; }
+        if constexpr (Op::DIAGE) {
+          // SparseFace: R selects the face nodes
+          const int nodeL = sm.perm[k][side == 0 ? f : fN];
+          const int nodeR = sm.perm[kR][side == 0 ? fN : f];
+          if (side == 0) {
+            m = k;
+            sign = -1.0;
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+              qL[c] = sm.q[el][nodeL][c];
+              qR[c] = Io<T>::load(qin, vin, (nb * NP + nodeR) * ND + c);
+            }
+          } else {
+            m = kR;
+            sign = 1.0;
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+              qL[c] = Io<T>::load(qin, vin, (nb * NP + nodeL) * ND + c);
+              qR[c] = sm.q[el][nodeR][c];
+            }
+          }
+        } else if (side == 0) {
+          // this element is elementL
+          m = k;
+          sign = -1.0;
+          for (int s = 0; s < ST; ++s) {
+            const double rl = sm.interp[s][k], rr = sm.interp[s][kR];
+            const int nodeL = sm.perm[s][f], nodeR = sm.perm[s][fN];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+              qL[c] = qL[c] + rl * sm.q[el][nodeL][c];
+              qR[c] = qR[c] + rr * Io<T>::load(qin, vin, (nb * NP + nodeR) * ND + c);
+            }
+          }
+        } else {
+          m = kR;
+          sign = 1.0;
+          for (int s = 0; s < ST; ++s) {
+            const double rl = sm.interp[s][k], rr = sm.interp[s][kR];
+            const int nodeL = sm.perm[s][fN], nodeR = sm.perm[s][f];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+              qL[c] = qL[c] + rl * Io<T>::load(qin, vin, (nb * NP + nodeL) * ND + c);
+              qR[c] = qR[c] + rr * sm.q[el][nodeR][c];
+            }
+          }
+        }
+        const double* nrm = mesh.nrm_face + ((int64_t)rec.y * NF + k) * DIM;
+        if constexpr (FLUX == PDES_FLUX_ROE) roe_flux<DIM>(phys, qL, qR, nrm, Fk);
+        else irslf_flux<DIM>(phys, qL, qR, nrm, Fk);
+      }
+      const double wgt = sign * sm.wface[m];
+#pragma unroll
+      for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = wgt * Fk[c];
+    }
+  }
+  __syncthreads();
+  if (active) {
+    // res_i += sum_f sum_m interp[s(i,f)][m] G[f][m]
+#pragma unroll
+    for (int f = 0; f < NFACES; ++f) {
+      const int s = sm.nodepos[i][f];
+      if (s < 0) continue;
+      if constexpr (Op::DIAGE) {
+#pragma unroll
+        for (int c = 0; c < ND; ++c) ri[c] = ri[c] + sm.G[el][f][s][c];
+      } else {
+#pragma unroll
+        for (int mm = 0; mm < NF; ++mm) {
+          const double r = sm.interp[s][mm];
+#pragma unroll
+          for (int c = 0; c < ND; ++c) ri[c] = ri[c] + r * sm.G[el][f][mm][c];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < ND; ++c) Io<T>::store(res, jv, (e * NP + i) * ND + c, ri[c]);
+  }
+}
+
+}  // namespace pdes

@@ -1,0 +1,354 @@
+"""Pins the CPU oracle against the reference's own known-answer tests, closed
+forms and invariants.  Each check names the reference test it restates."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from pdes_amd import abi, problems
+from helpers import setup
+
+PH = abi.Physics().desc()
+
+
+def _arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def pressure(oracle, q, dim=2):
+    f = oracle.lib().oracle_pressure
+    f.restype = C.c_double
+    q = _arr(q)
+    return f(dim, C.byref(PH), _p(q))
+
+
+def euler_flux(oracle, q, d):
+    q, d = _arr(q), _arr(d)
+    F = np.zeros(len(q))
+    oracle.lib().oracle_euler_flux(len(d), C.byref(PH), _p(q), _p(d), _p(F))
+    return F
+
+
+def roe(oracle, qL, qR, n):
+    qL, qR, n = _arr(qL), _arr(qR), _arr(n)
+    F = np.zeros(len(qL))
+    oracle.lib().oracle_roe(len(n), C.byref(PH), _p(qL), _p(qR), _p(n), _p(F))
+    return F
+
+
+def ir(oracle, qL, qR, dirs):
+    """dirs: (dim, ndir) -> F (ndir, ndof)"""
+    qL, qR = _arr(qL), _arr(qR)
+    d = _arr(dirs)
+    F = np.zeros((d.shape[1], len(qL)))
+    oracle.lib().oracle_ir_flux(d.shape[0], C.byref(PH), _p(qL), _p(qR), _p(d), d.shape[1], _p(F))
+    return F
+
+
+def irslf(oracle, qL, qR, n):
+    qL, qR, n = _arr(qL), _arr(qR), _arr(n)
+    F = np.zeros(len(qL))
+    oracle.lib().oracle_irslf_flux(len(n), C.byref(PH), _p(qL), _p(qR), _p(n), _p(F))
+    return F
+
+
+# ---------------------------------------------------------------- point functions
+def test_pressure_and_flux_known_answer(oracle):
+    # test/euler/test_lowlevel.jl:505-516: q = [1,2,3,7] -> p = 0.2, F = [2, 4.2, 6, 14.4]
+    q = [1.0, 2.0, 3.0, 7.0]
+    assert abs(pressure(oracle, q) - 0.2) < 1e-15
+    assert np.abs(euler_flux(oracle, q, [1.0, 0.0]) - [2.0, 4.2, 6.0, 14.4]).max() < 1e-14
+
+
+def test_isentropic_vortex_known_answer(oracle):
+    # test/euler/test_lowlevel.jl:645-649 at coords (1, 0)
+    sol = np.zeros(4)
+    coords = _arr([1.0, 0.0])
+    oracle.lib().oracle_vortex(C.byref(PH), _p(coords), _p(sol))
+    assert np.abs(sol - [2.0, 0.0, -1.3435, 2.236960]).max() < 1e-4
+    # closed form at r = 1: rho = 2, U = M_in * a, E = p/(g-1) + rho U^2 / 2
+    g = 1.4
+    p_in = 1 / g
+    a = math.sqrt(g * p_in / 2.0)
+    U = 0.95 * a
+    assert abs(sol[2] - (-2.0 * U)) < 1e-14
+    assert abs(sol[3] - (p_in / (g - 1) + 0.5 * 2.0 * U * U)) < 1e-14
+    # numpy IC restatement agrees (ICIsentropicVortex)
+    ph = abi.Physics()
+    pts = np.array([[1.0, 0.0], [1.7, 0.9], [0.3, 2.5]])
+    for x in pts:
+        oracle.lib().oracle_vortex(C.byref(PH), _p(_arr(x)), _p(sol))
+        assert np.abs(problems.vortex_exact(x[None, :], ph)[0] - sol).max() < 1e-14
+
+
+def test_roe_consistency_and_upwinding(oracle):
+    # test/euler/test_flux.jl:314-347 (testRoe) and test_dg.jl:31 consistency
+    qL = np.array([1.0, 2.0, 3.0, 7.0])
+    qR = qL + 1
+    n = np.array([1.0, 0.0])
+    assert np.abs(roe(oracle, qL, qR, n) - euler_flux(oracle, qL, n)).max() < 1e-13
+    assert np.abs(roe(oracle, qR, qL, n) - euler_flux(oracle, qR, n)).max() < 1e-13
+    assert np.abs(roe(oracle, qR, qL, n) + roe(oracle, qL, qR, -n)).max() < 1e-13
+    # indefinite flux Jacobian
+    qL = np.array([1.0, 1.0, 1.0, 7.0])
+    qR = qL + 1
+    assert np.abs(roe(oracle, qL, qR, n) + roe(oracle, qR, qL, -n)).max() < 1e-13
+    # consistency with a scaled oblique normal
+    for q in ([1.0, 0.3, -0.2, 2.5], [2.0, -1.0, 0.5, 5.0]):
+        nn = np.array([0.7, -1.3])
+        assert np.abs(roe(oracle, q, q, nn) - euler_flux(oracle, q, nn)).max() < 1e-13
+    # 3D
+    qL = np.array([1.0, 2.0, 3.0, 4.0, 15.0])
+    qR = qL + 1
+    n3 = np.array([1.0, 0.0, 0.0])
+    assert np.abs(roe(oracle, qL, qR, n3) - euler_flux(oracle, qL, n3)).max() < 1e-13
+    assert np.abs(roe(oracle, qR, qL, n3) + roe(oracle, qL, qR, -n3)).max() < 1e-13
+
+
+def _logmean_test(aL, aR):
+    # the reference test's own log mean (test_flux.jl:4-16)
+    xi = aL / aR
+    f = (xi - 1) / (xi + 1)
+    u = f * f
+    if u < 1e-2:
+        F = 1.0 + u / 3.0 + u * u / 5.0 + u * u * u / 7.0
+    else:
+        F = math.log(xi) / 2.0 / f
+    return (aL + aR) / (2 * F)
+
+
+def _ir_flux_test(qL, qR, nrm, g=1.4):
+    # independent restatement used by test_flux.jl:68-112
+    gm1 = g - 1
+    p = lambda q: gm1 * (q[3] - 0.5 * (q[1] ** 2 + q[2] ** 2) / q[0])
+    pL, pR = p(qL), p(qR)
+    z5 = _logmean_test(math.sqrt(qL[0] * pL), math.sqrt(qR[0] * pR))
+    z1L, z1R = math.sqrt(qL[0] / pL), math.sqrt(qR[0] / pR)
+    rho_hat = 0.5 * (z1L + z1R) * z5
+    z1a = 0.5 * (z1L + z1R)
+    u = 0.5 * (z1L * qL[1] / qL[0] + z1R * qR[1] / qR[0]) / z1a
+    v = 0.5 * (z1L * qL[2] / qL[0] + z1R * qR[2] / qR[0]) / z1a
+    p1 = 0.5 * (math.sqrt(qL[0] * pL) + math.sqrt(qR[0] * pR)) / z1a
+    z1 = _logmean_test(z1L, z1R)
+    p2 = (g + 1) * z5 / (2 * g * z1) + gm1 * 0.5 * (math.sqrt(qL[0] * pL) + math.sqrt(qR[0] * pR)) / (2 * g * z1a)
+    h = g * p2 / (rho_hat * gm1) + 0.5 * (u * u + v * v)
+    fx = np.array([rho_hat * u, rho_hat * u * u + p1, rho_hat * u * v, rho_hat * u * h])
+    fy = np.array([rho_hat * v, rho_hat * u * v, rho_hat * v * v + p1, rho_hat * v * h])
+    return fx * nrm[0] + fy * nrm[1]
+
+
+def test_ir_flux_against_independent_restatement(oracle):
+    qL = np.array([1.0, 2.0, 3.0, 7.0])
+    qR = qL + 1
+    nrm = np.array([1.0, 2.0])
+    F = ir(oracle, qL, qR, nrm[:, None])[0]
+    assert np.abs(F - _ir_flux_test(qL, qR, nrm)).max() < 1e-12
+
+
+def test_ir_flux_symmetry_consistency_multid(oracle):
+    # test_flux.jl:26-80 (symmetry, antisymmetry, consistency, multi-D)
+    qL = np.array([1.0, 2.0, 3.0, 7.0])
+    qR = qL + 1
+    n = np.array([[1.0], [1.0]])
+    F1 = ir(oracle, qL, qR, n)[0]
+    F2 = ir(oracle, qR, qL, n)[0]
+    assert np.abs(F1 - F2).max() < 1e-12
+    assert np.abs(F1 + ir(oracle, qR, qL, -n)[0]).max() < 1e-12
+    Fe = euler_flux(oracle, qL, n[:, 0])
+    assert np.abs(ir(oracle, qL, qL, n)[0] - Fe).max() < 1e-12
+    rng = np.random.default_rng(0)
+    D = rng.random((2, 2))
+    FD = ir(oracle, qL, qR, D)
+    for i in range(2):
+        assert np.abs(ir(oracle, qL, qR, D[:, i:i + 1])[0] - FD[i]).max() < 1e-14
+    # 3D consistency
+    q3 = np.array([1.0, 0.2, -0.3, 0.1, 3.0])
+    n3 = np.array([[0.3], [0.4], [-1.2]])
+    assert np.abs(ir(oracle, q3, q3, n3)[0] - euler_flux(oracle, q3, n3[:, 0])).max() < 1e-12
+
+
+def test_irslf_consistency_and_dissipation(oracle):
+    # IRSLF(q, q) = Euler flux (the kernel term vanishes) and the LF term
+    # dissipates entropy: (w_L - w_R).(F_IRSLF - F_IR) >= 0
+    qL = np.array([1.0, 0.3, -0.2, 2.5])
+    qR = np.array([1.1, 0.25, -0.1, 2.7])
+    n = np.array([0.6, 0.8])
+    assert np.abs(irslf(oracle, qL, qL, n) - euler_flux(oracle, qL, n)).max() < 1e-13
+    wL = np.zeros(4)
+    wR = np.zeros(4)
+    oracle.lib().oracle_ir_vars(2, C.byref(PH), _p(qL), _p(wL))
+    oracle.lib().oracle_ir_vars(2, C.byref(PH), _p(qR), _p(wR))
+    diss = irslf(oracle, qL, qR, n) - ir(oracle, qL, qR, n[:, None])[0]
+    assert np.dot(wL - wR, diss) > 0
+
+
+def test_logavg_branches_and_complex_step(oracle):
+    # logavg (bc_solvers.jl:942-956): both branches agree with the exact
+    # log mean; derivative by complex step vs analytic (test_jac.jl:2001-2025)
+    f = oracle.lib().oracle_logavg
+    f.restype = C.c_double
+    f.argtypes = [C.c_double, C.c_double]
+    for aL, aR in [(1.0, 2.0), (1.0, 1.0 + 1e-3), (3.0, 3.0000001), (0.5, 7.0)]:
+        # aR - aL is exact (Sterbenz) and log1p keeps the log accurate
+        exact = (aR - aL) / math.log1p((aR - aL) / aL)
+        assert abs(f(aL, aR) - exact) <= 4e-15 * exact
+    out = np.zeros(2)
+    h = 1e-20
+    for aL, aR in [(1.0, 2.0), (1.0, 1.001)]:
+        oracle.lib().oracle_logavg_complex(C.c_double(aL), C.c_double(h), C.c_double(aR), C.c_double(0.0), _p(out))
+        # d/daL of (aR - aL)/log(aR/aL)
+        L = math.log(aR / aL)
+        dexact = (-L + (aR - aL) / aL) / (L * L)
+        assert abs(out[1] / h - dexact) < 1e-10 * abs(dexact) + 1e-13
+
+
+# ---------------------------------------------------------------- residual invariants
+CASES = [("vortex", 4, 1, "omega", "roe"), ("vortex", 4, 2, "omega", "roe"),
+         ("vortex", 4, 3, "omega", "roe"), ("vortex", 3, 4, "omega", "roe"),
+         ("vortex", 4, 1, "diage", "roe"), ("vortex", 4, 2, "diage", "roe"),
+         ("vortex", 4, 2, "diage", "ir"), ("vortex", 3, 4, "diage", "ir"),
+         ("cube", 2, 1, "omega", "roe"), ("cube", 2, 2, "omega", "roe")]
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", CASES)
+def test_uniform_flow_zero_residual(oracle, kind, n, p, opk, flux):
+    # test/euler/test_dg.jl:112-131: uniform flow with freestream BC -> res = 0
+    m, op, ph, b = setup(kind, n, p, opk, flux, Ma=0.4, aoa_deg=10.0)
+    ph.bc_type = [abi.PDES_BC_FREESTREAM] * m.numBC
+    b = abi.make_descriptors(m, op, ph)
+    q = problems.freestream(m.coords.shape[:-1], m.dim, ph)
+    assert np.abs(oracle.residual(b, q)).max() < 1e-13
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", CASES)
+def test_conservation(oracle, kind, n, p, opk, flux):
+    # sum of each conserved variable's residual (weighted by 1) equals minus the
+    # boundary flux integral: interior face and volume terms telescope
+    m, op, ph, b = setup(kind, n, p, opk, flux)
+    q = problems.vortex_noise_state(m, ph) if kind == "vortex" else problems.freestream_bump_state(m, ph)
+    r = oracle.residual(b, q)
+    # recompute with the boundary removed: turn every face into its own
+    # contribution via a residual of a state with zero boundary flux is not
+    # available, so compare the total against the oracle boundary flux sum
+    tot = r.sum(axis=(0, 1))
+    bsum = _boundary_flux_sum(oracle, m, op, ph, q)
+    assert np.abs(tot + bsum).max() < 1e-12 * (1 + np.abs(bsum).max())
+
+
+def _boundary_flux_sum(oracle, m, op, ph, q):
+    f = op.face
+    nd = m.dim + 2
+    tot = np.zeros(nd)
+    for bc in range(m.numBC):
+        for bi in range(m.bndry_offsets[bc], m.bndry_offsets[bc + 1]):
+            e, fc = m.bndryfaces[bi]
+            qe = q[e]
+            qb = f.interp.T @ qe[f.perm[:, fc]]
+            for k in range(f.numnodes):
+                if ph.bc_type[bc] == abi.PDES_BC_ISENTROPIC_VORTEX:
+                    qg = problems.vortex_exact(m.coords_bndry[bi, k][None, :], ph)[0]
+                else:
+                    qg = problems.freestream((), m.dim, ph)
+                Fk = roe(oracle, qb[k], qg, m.nrm_bndry[bi, k]) if m.dim == 2 else None
+                if m.dim == 3:
+                    Fk = np.zeros(nd)
+                    oracle.lib().oracle_roe(3, C.byref(PH), _p(_arr(qb[k])), _p(_arr(qg)),
+                                            _p(_arr(m.nrm_bndry[bi, k])), _p(Fk))
+                tot += f.wface[k] * Fk
+    return tot
+
+
+def test_absvalue_kink_semantics(oracle):
+    """absvalue (complexify.jl:25-45) flips by the sign of the real part, so at
+    Un = 0 exactly the complex step returns the right-sided derivative: with
+    rho*v = 0 on a y-normal face the derivative of the Roe flux w.r.t. rho*v
+    equals the one-sided finite difference from above, not the central one."""
+    qL = np.array([1.0, 0.5, 0.0, -0.02, 2.0])
+    qR = np.array([1.0, 0.5, 0.0, -0.02, 2.0]) * 1.01
+    n = np.array([0.0, -0.25, 0.0])
+    v = np.zeros(5)
+    v[2] = 1.0
+    Fr, Fi = np.zeros(5), np.zeros(5)
+    h = 1e-20
+    oracle.lib().oracle_roe_complex(3, C.byref(PH), _p(qL), _p(_arr(h * v)), _p(qR), _p(np.zeros(5)),
+                                    _p(n), _p(Fr), _p(Fi))
+    def F(x):
+        out = np.zeros(5)
+        oracle.lib().oracle_roe(3, C.byref(PH), _p(_arr(x)), _p(qR), _p(n), _p(out))
+        return out
+    # Un = v * ny with ny < 0: the branch absvalue(Un) = Un (Un >= 0) is the
+    # one reached by a step in -v
+    fwd = (F(qL - 1e-8 * v) - F(qL)) / (-1e-8)
+    assert np.abs(Fi / h - fwd).max() < 1e-6 * (1 + np.abs(fwd).max())
+
+
+def test_checkpositivity(oracle):
+    m, op, ph, b = setup("vortex", 3, 1, "omega", "roe")
+    q = problems.vortex_noise_state(m, ph)
+    q[2, 1, 0] = -1.0
+    assert oracle.residual_status(b, q) == -4
+
+
+def test_complex_step_jv_matches_finite_difference(oracle):
+    # evaldRdqProduct vs a central difference (the complex step is exact to
+    # rounding; the FD error is O(h^2))
+    m, op, ph, b = setup("vortex", 4, 2, "diage", "ir")
+    q = problems.vortex_noise_state(m, ph)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(q.shape)
+    jv = oracle.jv_complex_step(b, q, v)
+    h = 1e-6
+    fd = (oracle.residual(b, q + h * v) - oracle.residual(b, q - h * v)) / (2 * h)
+    assert np.abs(jv - fd).max() < 1e-7 * (1 + np.abs(jv).max())
+
+
+def test_sparsity_symmetric_and_complete(oracle):
+    # structurally symmetric, sorted, no duplicates; coloring is distance-2
+    for kind, n, p, opk in [("vortex", 4, 1, "omega"), ("vortex", 4, 2, "diage"), ("cube", 2, 2, "omega")]:
+        m, op, ph, b = setup(kind, n, p, opk, "roe")
+        rowptr, colind = oracle.sparsity_bsr(b)
+        Np = op.numnodes
+        nnz_expected = m.numEl * Np * Np + 2 * m.numInterfaces * (
+            op.face.stencilsize ** 2 if op.face.dense else op.face.numnodes)
+        assert rowptr[-1] == nnz_expected
+        pairs = set()
+        for r in range(len(rowptr) - 1):
+            cols = colind[rowptr[r]:rowptr[r + 1]]
+            assert np.all(np.diff(cols) > 0)
+            for c in cols:
+                pairs.add((r, int(c)))
+        for (r, c) in pairs:
+            assert (c, r) in pairs
+        color, nc = oracle.color_elements(b)
+        nb = [set() for _ in range(m.numEl)]
+        for (eL, eR, *_r) in m.interfaces:
+            nb[eL].add(eR)
+            nb[eR].add(eL)
+        for e in range(m.numEl):
+            d2 = set(nb[e])
+            for a in nb[e]:
+                d2 |= nb[a]
+            d2.discard(e)
+            assert all(color[x] != color[e] for x in d2)
+
+
+def test_complex_step_jacobian_matches_jv(oracle):
+    # test_jac.jl:2370-2457: explicit matrix times v equals the complex-step Jv
+    import scipy.sparse as sp
+    for kind, n, p, opk, fl in [("vortex", 3, 1, "omega", "roe"), ("vortex", 3, 2, "diage", "ir"),
+                                ("cube", 1, 1, "omega", "roe")]:
+        m, op, ph, b = setup(kind, n, p, opk, fl)
+        q = problems.vortex_noise_state(m, ph) if kind == "vortex" else problems.freestream_bump_state(m, ph)
+        rowptr, colind = oracle.sparsity_bsr(b)
+        val = oracle.jacobian_cs(b, q, rowptr, colind)
+        nd = m.dim + 2
+        nrow = m.numEl * op.numnodes
+        J = sp.bsr_matrix((val, colind, rowptr), shape=(nrow * nd, nrow * nd))
+        v = np.random.default_rng(1).standard_normal(q.size)
+        jv = oracle.jv_complex_step(b, q, v.reshape(q.shape)).ravel()
+        assert np.abs(J @ v - jv).max() < 1e-11 * (1 + np.abs(jv).max())
