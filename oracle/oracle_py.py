@@ -109,3 +109,69 @@ def jacobian_cs(b, q, rowptr, colind):
                                 q.ctypes.data, rowptr.ctypes.data, colind.ctypes.data,
                                 val.ctypes.data), "jacobian_cs")
     return val
+
+
+def newton_direct(b, q0, Minv, res_abstol=1e-10, itermax=30):
+    """newtonInner (src/NonlinearSolvers/newton.jl:137-304) with the sparse
+    direct solve of jac_type 2 (UMFPACK in the reference, SuperLU here):
+    J dq = -R, q += dq until ||R||_{M^-1} < res_abstol.  Returns (q, norms)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    rowptr, colind = sparsity_bsr(b)
+    nd = b.mesh.dim + 2
+    nrow = b.mesh.numEl * b.sbp.Np
+    q = np.array(q0, dtype=np.float64, copy=True)
+    norms = []
+    for it in range(itermax + 1):
+        r = residual(b, q)
+        nrm = float(np.sqrt((r * r * Minv).sum()))
+        norms.append(nrm)
+        if nrm < res_abstol or it == itermax:
+            break
+        val = jacobian_cs(b, q, rowptr, colind)
+        J = sp.bsr_matrix((val, colind, rowptr), shape=(nrow * nd, nrow * nd)).tocsc()
+        dq = spla.spsolve(J, -r.ravel())
+        q = q + dq.reshape(q.shape)
+    return q, norms
+
+
+def functional(m, op, ph, q, bcnums):
+    """calcBndryFunctional (src/solver/euler/boundary_functional.jl:82-126) for
+    the force, entropy-flux and mass-flow integrands (:340-436), summed over
+    the faces of the listed BC groups.  Returns dict of values."""
+    g = ph.gamma
+    f = op.face
+    dim = m.dim
+    force = np.zeros(dim)
+    ent = 0.0
+    mass = 0.0
+    for bc in bcnums:
+        for bi in range(m.bndry_offsets[bc], m.bndry_offsets[bc + 1]):
+            e, fc = m.bndryfaces[bi]
+            qb = f.interp.T @ q[e][f.perm[:, fc]]
+            for k in range(f.numnodes):
+                nrm = m.nrm_bndry[bi, k]
+                w = f.wface[k]
+                qk = qb[k]
+                nh = nrm / np.sqrt((nrm * nrm).sum())
+                nm = (nh * qk[1:dim + 1]).sum()
+                qg = qk.copy()
+                qg[1:dim + 1] -= nh * nm
+                p = (g - 1) * (qg[-1] - 0.5 * (qg[1:dim + 1] ** 2).sum() / qg[0])
+                U = (qg[1:dim + 1] * nrm).sum() / qg[0]
+                F = np.concatenate([[qg[0] * U], qg[1:dim + 1] * U + nrm * p, [(qg[-1] + p) * U]])
+                force += w * F[1:dim + 1]
+                unet = (qk[1:dim + 1] * nrm).sum()
+                pk = (g - 1) * (qk[-1] - 0.5 * (qk[1:dim + 1] ** 2).sum() / qk[0])
+                s = np.log(pk) - g * np.log(qk[0])
+                Uent = -qk[0] * s / (g - 1)
+                ent += w * unet * Uent
+                mass += w * unet
+    aoa = ph.aoa_deg * np.pi / 180
+    if dim == 2:
+        lift = -np.sin(aoa) * force[0] + np.cos(aoa) * force[1]
+        drag = np.cos(aoa) * force[0] + np.sin(aoa) * force[1]
+    else:
+        lift = -np.sin(aoa) * force[0] + np.cos(aoa) * force[2]
+        drag = np.cos(aoa) * force[0] + np.sin(aoa) * force[2]
+    return {"force": force, "lift": lift, "drag": drag, "entropy": ent, "mass": mass}

@@ -47,6 +47,7 @@ struct pdes_ctx {
   // solver state
   pdes_jacobian* jac = nullptr;
   pdes_precond* pc_obj = nullptr;
+  void* gmres_ws = nullptr;     // cached GMRES workspace
 
   pdes::DevMesh devmesh() const {
     return pdes::DevMesh{numEl, d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};

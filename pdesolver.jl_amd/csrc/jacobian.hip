@@ -1,0 +1,212 @@
+// jacobian.hip -- BSR pattern, explicit Jacobian assembly and BSR products (C ABI).
+#include <algorithm>
+
+#include "jacobian.hpp"
+
+namespace pdes {
+
+int jacobian_free(pdes_ctx* c) {
+  if (!c->jac) return PDES_OK;
+  pdes_jacobian* j = c->jac;
+  if (j->d_rowptr) hipFree(j->d_rowptr);
+  if (j->d_colind) hipFree(j->d_colind);
+  if (j->d_vals) hipFree(j->d_vals);
+  if (j->d_err) hipFree(j->d_err);
+  delete j;
+  c->jac = nullptr;
+  return PDES_OK;
+}
+
+// Block sparsity of SparseMatrixCSC(mesh, Tv, INVISCID, face_type)
+// (src/jacobian/sparse.jl:250-495): every node couples to all nodes of its
+// element; across an interface, DenseFace couples all stencil nodes of both
+// sides, SparseFace couples each face node with its nbrperm partner.
+static void build_pattern(const pdes_ctx* c, std::vector<int64_t>& rowptr, std::vector<int32_t>& colind) {
+  const int Np = c->Np, Nf = c->Nf, nfaces = c->nfaces;
+  const int64_t nrows = (int64_t)c->numEl * Np;
+  // inverse nbrperm per orientation
+  std::vector<int32_t> inv((size_t)Nf * c->norient);
+  for (int o = 0; o < c->norient; ++o)
+    for (int k = 0; k < Nf; ++k) inv[(size_t)c->h_nbrperm[(size_t)k * c->norient + o] * c->norient + o] = k;
+  rowptr.assign(nrows + 1, 0);
+  std::vector<int32_t> cols;
+  cols.reserve(Np * (nfaces + 1));
+  std::vector<std::vector<int32_t>> rows_tmp;  // unused; rows are built in order
+  colind.clear();
+  for (int e = 0; e < c->numEl; ++e) {
+    for (int i = 0; i < Np; ++i) {
+      cols.clear();
+      for (int j = 0; j < Np; ++j) cols.push_back(e * Np + j);
+      for (int f = 0; f < nfaces; ++f) {
+        const ElFace r = c->h_elface[(size_t)e * nfaces + f];
+        if (r.x < 0) continue;
+        const int pos = c->h_nodepos[(size_t)i * nfaces + f];
+        if (pos < 0) continue;
+        const int fN = r.z & 0xff, o = (r.z >> 8) & 0xff, side = (r.z >> 16) & 1;
+        if (!c->diag_e) {
+          for (int s = 0; s < c->stencil; ++s) cols.push_back(r.x * Np + c->h_perm[(size_t)s * nfaces + fN]);
+        } else {
+          // own face node pos; partner face node on the neighbour
+          int kn;
+          if (side == 0) kn = c->h_nbrperm[(size_t)pos * c->norient + o];
+          else kn = inv[(size_t)pos * c->norient + o];
+          cols.push_back(r.x * Np + c->h_perm[(size_t)kn * nfaces + fN]);
+        }
+      }
+      std::sort(cols.begin(), cols.end());
+      const int64_t row = (int64_t)e * Np + i;
+      rowptr[row + 1] = rowptr[row] + (int64_t)cols.size();
+      colind.insert(colind.end(), cols.begin(), cols.end());
+    }
+  }
+}
+
+template <class Op, int VOL, int FLUX, class BT>
+static int launch_jac(pdes_ctx* c) {
+  pdes_jacobian* j = c->jac;
+  hipLaunchKernelGGL((euler_jacobian_kernel<Op, VOL, FLUX, BT>), dim3(c->numEl), dim3(256), 0, c->stream,
+                     c->devmesh(), c->devsbp(), c->pc, c->d_q, j->d_rowptr, j->d_colind,
+                     (BT*)j->d_vals, j->d_err);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+template <class Op, class BT>
+static int launch_jac_shape(pdes_ctx* c) {
+  const int vol = c->phys.volume_integral_type, flux = c->phys.face_flux;
+  if (vol == 1 && flux == PDES_FLUX_ROE) return launch_jac<Op, VOL_WEAK, PDES_FLUX_ROE, BT>(c);
+  if constexpr (Op::DIAGE && Op::DIM == 2) {
+    if (vol == 2 && flux == PDES_FLUX_IRSLF) return launch_jac<Op, VOL_SPLIT, PDES_FLUX_IRSLF, BT>(c);
+  }
+  set_error("unsupported physics combination for the explicit Jacobian");
+  return PDES_ERR_UNSUPPORTED;
+}
+
+int launch_assemble(pdes_ctx* c) {
+#define PDES_DISPATCH(D, NP, NF, ST, DE)                                                        \
+  if (c->dim == D && c->Np == NP && c->Nf == NF && c->stencil == ST && (c->diag_e != 0) == DE) { \
+    using Op = OpShape<D, NP, NF, ST, DE>;                                                      \
+    if (c->jac->fp64) return launch_jac_shape<Op, double>(c);                                   \
+    return launch_jac_shape<Op, float>(c);                                                      \
+  }
+  PDES_FOR_EACH_SHAPE(PDES_DISPATCH)
+#undef PDES_DISPATCH
+  set_error("no compiled Jacobian kernel for this operator");
+  return PDES_ERR_UNSUPPORTED;
+}
+
+template <int ND>
+static int launch_spmv_nd(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta) {
+  pdes_jacobian* j = c->jac;
+  const int64_t n = j->nrows * ND;
+  const int threads = 256;
+  const int64_t blocks = ceil_div(n, threads);
+  if (j->fp64)
+    hipLaunchKernelGGL((bsr_spmv_kernel<ND, double>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, j->nrows,
+                       j->d_rowptr, j->d_colind, (const double*)j->d_vals, x, y, alpha, yadd, beta);
+  else
+    hipLaunchKernelGGL((bsr_spmv_kernel<ND, float>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, j->nrows,
+                       j->d_rowptr, j->d_colind, (const float*)j->d_vals, x, y, alpha, yadd, beta);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+int launch_spmv(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta) {
+  PDES_CHECK(c->jac && c->jac->assembled, PDES_ERR_ARG, "Jacobian not assembled");
+  if (c->nd == 4) return launch_spmv_nd<4>(c, x, y, alpha, yadd, beta);
+  return launch_spmv_nd<5>(c, x, y, alpha, yadd, beta);
+}
+
+int jacobian_setup(pdes_ctx* c, int fp64) {
+  if (c->jac && c->jac->fp64 == (fp64 != 0)) return PDES_OK;
+  jacobian_free(c);
+  pdes_jacobian* j = new pdes_jacobian();
+  c->jac = j;
+  j->fp64 = fp64 != 0;
+  std::vector<int64_t> rowptr;
+  std::vector<int32_t> colind;
+  build_pattern(c, rowptr, colind);
+  j->nrows = (int64_t)c->numEl * c->Np;
+  j->nnzb = rowptr.back();
+  j->h_rowptr = rowptr;
+  j->h_colind = colind;
+  const size_t blk = (size_t)c->nd * c->nd * (j->fp64 ? 8 : 4);
+  hipError_t e1 = hipMalloc((void**)&j->d_rowptr, rowptr.size() * sizeof(int64_t));
+  hipError_t e2 = hipMalloc((void**)&j->d_colind, std::max<size_t>(1, colind.size()) * sizeof(int32_t));
+  hipError_t e3 = hipMalloc(&j->d_vals, std::max<size_t>(1, (size_t)j->nnzb) * blk);
+  hipError_t e4 = hipMalloc((void**)&j->d_err, sizeof(int32_t));
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+    set_error("device allocation of the BSR matrix (%lld blocks) failed", (long long)j->nnzb);
+    jacobian_free(c);
+    return PDES_ERR_NOMEM;
+  }
+  PDES_HIP(hipMemcpyAsync(j->d_rowptr, rowptr.data(), rowptr.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipMemcpyAsync(j->d_colind, colind.data(), colind.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipMemsetAsync(j->d_err, 0, sizeof(int32_t), c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int assemble(pdes_ctx* c) {
+  PDES_CHECK(c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  int st = launch_assemble(c);
+  if (st) return st;
+  c->jac->assembled = true;
+  return PDES_OK;
+}
+
+}  // namespace pdes
+
+using namespace pdes;
+
+extern "C" {
+
+int pdes_jacobian_setup(pdes_ctx* c, int fp64_blocks) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  return jacobian_setup(c, fp64_blocks);
+}
+
+int pdes_jacobian_nnzb(pdes_ctx* c, int64_t* nrows_blk, int64_t* nnzb) {
+  PDES_CHECK(c && c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  if (nrows_blk) *nrows_blk = c->jac->nrows;
+  if (nnzb) *nnzb = c->jac->nnzb;
+  return PDES_OK;
+}
+
+int pdes_get_bsr_pattern(pdes_ctx* c, int64_t* rowptr, int32_t* colind) {
+  PDES_CHECK(c && c->jac && rowptr && colind, PDES_ERR_ARG, "bad argument");
+  std::copy(c->jac->h_rowptr.begin(), c->jac->h_rowptr.end(), rowptr);
+  std::copy(c->jac->h_colind.begin(), c->jac->h_colind.end(), colind);
+  return PDES_OK;
+}
+
+int pdes_assemble_jacobian(pdes_ctx* c) {
+  PDES_CHECK(c && c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  int st = assemble(c);
+  if (st) return st;
+  int32_t err = 0;
+  PDES_HIP(hipMemcpyAsync(&err, c->jac->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  PDES_CHECK(err == 0, PDES_ERR_ARG, "a Jacobian block fell outside the sparsity pattern");
+  return PDES_OK;
+}
+
+int pdes_get_bsr_values(pdes_ctx* c, void* values) {
+  PDES_CHECK(c && c->jac && c->jac->assembled && values, PDES_ERR_ARG, "Jacobian not assembled");
+  const size_t bytes = (size_t)c->jac->nnzb * c->nd * c->nd * (c->jac->fp64 ? 8 : 4);
+  PDES_HIP(hipMemcpyAsync(values, c->jac->d_vals, bytes, hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_bsr_matvec(pdes_ctx* c, const double* x, double* b) {
+  PDES_CHECK(c && x && b, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(c->d_work, x, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = launch_spmv(c, c->d_work, c->d_work2, 1.0, nullptr, 0.0);
+  if (st) return st;
+  PDES_HIP(hipMemcpyAsync(b, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,25 @@
+// jacobian.hpp -- device BSR Jacobian owned by a context.
+#pragma once
+#include "ctx.hpp"
+#include "jacobian_kernels.hpp"
+#include "shapes.hpp"
+
+struct pdes_jacobian {
+  bool fp64 = false;
+  bool assembled = false;
+  int64_t nrows = 0;   // block rows (nodes)
+  int64_t nnzb = 0;
+  int64_t* d_rowptr = nullptr;
+  int32_t* d_colind = nullptr;
+  void* d_vals = nullptr;    // float or double blocks [nnzb][nd][nd], row-major
+  int32_t* d_err = nullptr;
+  std::vector<int64_t> h_rowptr;
+  std::vector<int32_t> h_colind;
+};
+
+namespace pdes {
+int jacobian_setup(pdes_ctx* c, int fp64);
+int assemble(pdes_ctx* c);
+// y = alpha J x + beta yadd (yadd may be null)
+int launch_spmv(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta);
+}  // namespace pdes

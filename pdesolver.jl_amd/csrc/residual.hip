@@ -1,22 +1,9 @@
 // residual.hip -- operator-shape dispatch of the fused residual kernel and the
 // reduction / vector kernels used around it.
 #include "ctx.hpp"
+#include "shapes.hpp"
 
 namespace pdes {
-
-// Operator shapes compiled for gfx950: triangle Omega p=1..4, triangle
-// diagonal-E p=1..4, tetrahedron Omega p=1..2.
-#define PDES_FOR_EACH_SHAPE(X)          \
-  X(2, 3, 2, 3, false)                  \
-  X(2, 6, 3, 6, false)                  \
-  X(2, 10, 4, 10, false)                \
-  X(2, 15, 5, 15, false)                \
-  X(2, 7, 3, 3, true)                   \
-  X(2, 12, 4, 4, true)                  \
-  X(2, 18, 5, 5, true)                  \
-  X(2, 27, 6, 6, true)                  \
-  X(3, 4, 3, 4, false)                  \
-  X(3, 10, 6, 10, false)
 
 template <class Op, class T>
 constexpr int epb_for() {

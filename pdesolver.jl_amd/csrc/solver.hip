@@ -1,0 +1,607 @@
+// solver.hip -- preconditioners, GMRES and Newton on the device (C ABI).
+//
+// NewtonBDiagPC (src/NonlinearSolvers/preconditioning_types.jl:41-110,
+// preconditioning.jl:7-215): LU of each element's diagonal (Np*ndof)^2 block of
+// the Jacobian, applied element by element.  Factor and apply run in FP32.
+// GMRES(restart) with right preconditioning replaces the PETSc KSP of
+// src/linearsolvers/ls_standard.jl:321-370 (KSP options read_input.jl:560-581);
+// the Arnoldi basis, Hessenberg and update stay FP64.  Orthogonalisation is
+// classical Gram-Schmidt applied twice (CGS2): two multi-dot and two
+// multi-axpy passes per step, each reading the basis once.
+// Newton follows newtonInner (src/NonlinearSolvers/newton.jl:137-304).
+#include <chrono>
+#include <cmath>
+
+#include "jacobian.hpp"
+
+struct pdes_precond {
+  int kind = 0;
+  int n = 0;                 // block size Np*nd (block-Jacobi)
+  float* d_lu = nullptr;     // [numEl][n][n] column-major LU
+  int32_t* d_piv = nullptr;  // [numEl][n]
+  bool factored = false;
+};
+
+namespace pdes {
+
+struct GmresWork;
+void gmres_release(pdes_ctx* c);
+
+int precond_free(pdes_ctx* c) {
+  gmres_release(c);
+  if (!c->pc_obj) return PDES_OK;
+  if (c->pc_obj->d_lu) hipFree(c->pc_obj->d_lu);
+  if (c->pc_obj->d_piv) hipFree(c->pc_obj->d_piv);
+  delete c->pc_obj;
+  c->pc_obj = nullptr;
+  return PDES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// block-Jacobi factor: gather the element block from the BSR matrix into LDS
+// (FP32), LU with partial pivoting, store column-major.
+// ---------------------------------------------------------------------------
+template <int NP, int ND, class BT>
+__global__ void __launch_bounds__(256)
+bjacobi_factor_kernel(int numEl, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+                      const BT* __restrict__ vals, float* __restrict__ lu, int32_t* __restrict__ piv,
+                      int32_t* __restrict__ err) {
+  constexpr int N = NP * ND;
+  __shared__ float A[N][N + 1];
+  __shared__ float redv[64];
+  __shared__ int redi[64];
+  __shared__ int pivrow;
+  const int tid = threadIdx.x;
+  const int64_t e = blockIdx.x;
+  for (int t = tid; t < NP * NP; t += blockDim.x) {
+    const int i = t / NP, j = t - i * NP;
+    const int64_t row = e * NP + i;
+    const int64_t pos = bsr_find(rowptr, colind, row, (int32_t)(e * NP + j));
+    for (int a = 0; a < ND; ++a)
+      for (int b = 0; b < ND; ++b)
+        A[i * ND + a][j * ND + b] = pos >= 0 ? (float)vals[pos * ND * ND + a * ND + b] : 0.0f;
+  }
+  __syncthreads();
+  for (int k = 0; k < N; ++k) {
+    // pivot: max |A[r][k]|, r >= k (first wave)
+    if (tid < 64) {
+      float best = -1.0f;
+      int bi = k;
+      for (int r = k + tid; r < N; r += 64) {
+        float v = fabsf(A[r][k]);
+        if (v > best) { best = v; bi = r; }
+      }
+      redv[tid] = best;
+      redi[tid] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float best = -1.0f;
+      int bi = k;
+      for (int t = 0; t < 64; ++t)
+        if (redv[t] > best || (redv[t] == best && redi[t] < bi)) { best = redv[t]; bi = redi[t]; }
+      pivrow = bi;
+      piv[e * N + k] = bi;
+      if (!(best > 0.0f)) err[0] = 1;
+    }
+    __syncthreads();
+    const int p = pivrow;
+    if (p != k)
+      for (int cc = tid; cc < N; cc += blockDim.x) {
+        float t0 = A[k][cc];
+        A[k][cc] = A[p][cc];
+        A[p][cc] = t0;
+      }
+    __syncthreads();
+    const float dinv = 1.0f / A[k][k];
+    for (int r = k + 1 + tid; r < N; r += blockDim.x) A[r][k] *= dinv;
+    __syncthreads();
+    const int m = N - k - 1;
+    for (int t = tid; t < m * m; t += blockDim.x) {
+      const int r = k + 1 + t / m, cc = k + 1 + t % m;
+      A[r][cc] -= A[r][k] * A[k][cc];
+    }
+    __syncthreads();
+  }
+  float* out = lu + e * N * N;
+  for (int t = tid; t < N * N; t += blockDim.x) {
+    const int cc = t / N, r = t - cc * N;
+    out[t] = A[r][cc];
+  }
+}
+
+// block-Jacobi apply: one wave per element, lane r owns rows r, r+64, ...
+template <int NP, int ND>
+__global__ void __launch_bounds__(64)
+bjacobi_apply_kernel(int numEl, const float* __restrict__ lu, const int32_t* __restrict__ piv,
+                     const double* __restrict__ b, double* __restrict__ x) {
+  constexpr int N = NP * ND;
+  __shared__ float v[N];
+  const int lane = threadIdx.x;
+  const int64_t e = blockIdx.x;
+  for (int r = lane; r < N; r += 64) v[r] = (float)b[e * N + r];
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < N; ++k) {
+      const int p = piv[e * N + k];
+      if (p != k) { float t = v[k]; v[k] = v[p]; v[p] = t; }
+    }
+  __syncthreads();
+  const float* L = lu + e * N * N;
+  for (int k = 0; k < N; ++k) {
+    const float xk = v[k];
+    for (int r = k + 1 + lane; r < N; r += 64) v[r] -= L[k * N + r] * xk;
+    __syncthreads();
+  }
+  for (int k = N - 1; k >= 0; --k) {
+    if (lane == 0) v[k] = v[k] / L[k * N + k];
+    __syncthreads();
+    const float xk = v[k];
+    for (int r = lane; r < k; r += 64) v[r] -= L[k * N + r] * xk;
+    __syncthreads();
+  }
+  for (int r = lane; r < N; r += 64) x[e * N + r] = (double)v[r];
+}
+
+template <int NP, int ND>
+static int bj_factor(pdes_ctx* c) {
+  pdes_jacobian* j = c->jac;
+  pdes_precond* p = c->pc_obj;
+  if (j->fp64)
+    hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, double>), dim3(c->numEl), dim3(256), 0, c->stream, c->numEl,
+                       j->d_rowptr, j->d_colind, (const double*)j->d_vals, p->d_lu, p->d_piv, j->d_err);
+  else
+    hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, float>), dim3(c->numEl), dim3(256), 0, c->stream, c->numEl,
+                       j->d_rowptr, j->d_colind, (const float*)j->d_vals, p->d_lu, p->d_piv, j->d_err);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+template <int NP, int ND>
+static int bj_apply(pdes_ctx* c, const double* b, double* x) {
+  hipLaunchKernelGGL((bjacobi_apply_kernel<NP, ND>), dim3(c->numEl), dim3(64), 0, c->stream, c->numEl,
+                     c->pc_obj->d_lu, c->pc_obj->d_piv, b, x);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+static int pc_factor(pdes_ctx* c) {
+  if (c->dim == 2) {
+    switch (c->Np) {
+      case 3: return bj_factor<3, 4>(c);
+      case 6: return bj_factor<6, 4>(c);
+      case 10: return bj_factor<10, 4>(c);
+      case 15: return bj_factor<15, 4>(c);
+      case 7: return bj_factor<7, 4>(c);
+      case 12: return bj_factor<12, 4>(c);
+      case 18: return bj_factor<18, 4>(c);
+      case 27: return bj_factor<27, 4>(c);
+    }
+  } else {
+    switch (c->Np) {
+      case 4: return bj_factor<4, 5>(c);
+      case 10: return bj_factor<10, 5>(c);
+    }
+  }
+  set_error("no block-Jacobi kernel for this operator");
+  return PDES_ERR_UNSUPPORTED;
+}
+
+__global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+int pc_apply(pdes_ctx* c, const double* b, double* x) {
+  if (!c->pc_obj || c->pc_obj->kind == PDES_PC_NONE) {
+    const int64_t n = c->numDof;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)std::min<int64_t>(8192, ceil_div(n, 256))), dim3(256), 0,
+                       c->stream, b, x, n);
+    PDES_HIP(hipGetLastError());
+    return PDES_OK;
+  }
+  if (c->dim == 2) {
+    switch (c->Np) {
+      case 3: return bj_apply<3, 4>(c, b, x);
+      case 6: return bj_apply<6, 4>(c, b, x);
+      case 10: return bj_apply<10, 4>(c, b, x);
+      case 15: return bj_apply<15, 4>(c, b, x);
+      case 7: return bj_apply<7, 4>(c, b, x);
+      case 12: return bj_apply<12, 4>(c, b, x);
+      case 18: return bj_apply<18, 4>(c, b, x);
+      case 27: return bj_apply<27, 4>(c, b, x);
+    }
+  } else {
+    switch (c->Np) {
+      case 4: return bj_apply<4, 5>(c, b, x);
+      case 10: return bj_apply<10, 5>(c, b, x);
+    }
+  }
+  set_error("no block-Jacobi kernel for this operator");
+  return PDES_ERR_UNSUPPORTED;
+}
+
+int pc_setup(pdes_ctx* c, int kind) {
+  PDES_CHECK(kind == PDES_PC_NONE || kind == PDES_PC_BJACOBI, PDES_ERR_UNSUPPORTED,
+             "preconditioner kind %d is not available", kind);
+  if (!c->pc_obj) c->pc_obj = new pdes_precond();
+  pdes_precond* p = c->pc_obj;
+  p->kind = kind;
+  if (kind == PDES_PC_NONE) { p->factored = true; return PDES_OK; }
+  PDES_CHECK(c->jac && c->jac->assembled, PDES_ERR_ARG, "assemble the Jacobian before the preconditioner");
+  const int n = c->Np * c->nd;
+  if (!p->d_lu || p->n != n) {
+    if (p->d_lu) hipFree(p->d_lu);
+    if (p->d_piv) hipFree(p->d_piv);
+    p->n = n;
+    if (hipMalloc((void**)&p->d_lu, (size_t)c->numEl * n * n * sizeof(float)) != hipSuccess ||
+        hipMalloc((void**)&p->d_piv, (size_t)c->numEl * n * sizeof(int32_t)) != hipSuccess) {
+      set_error("device allocation of the block-Jacobi factors failed");
+      return PDES_ERR_NOMEM;
+    }
+  }
+  int st = pc_factor(c);
+  if (st) return st;
+  p->factored = true;
+  return PDES_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES building blocks
+// ---------------------------------------------------------------------------
+constexpr int MD_THREADS = 256;
+constexpr int MD_BLOCKS = 512;
+
+// partial[i][b] = sum over block b's slice of V_i . w
+__global__ void __launch_bounds__(MD_THREADS)
+multidot_partial(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ w, int64_t n,
+                 double* __restrict__ partial) {
+  __shared__ double sh[MD_THREADS];
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int i = 0; i < m; ++i) {
+    double acc = 0.0;
+    const double* Vi = V + (int64_t)i * ldv;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) acc += Vi[t] * w[t];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)i * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void multidot_reduce(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  __shared__ double sh[256];
+  const int i = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += partial[(int64_t)i * nb + b];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = sh[0];
+}
+
+// w -= sum_i h[i] V_i
+__global__ void multiaxpy(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ h,
+                          double* __restrict__ w, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    double acc = w[t];
+    for (int i = 0; i < m; ++i) acc -= h[i] * V[(int64_t)i * ldv + t];
+    w[t] = acc;
+  }
+}
+
+// y = a*x (+ y if accumulate)
+__global__ void scale_kernel(const double* __restrict__ x, double* __restrict__ y, double a, int accumulate, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    y[t] = accumulate ? y[t] + a * x[t] : a * x[t];
+}
+
+// y = sum_i c[i] V_i  (c on the host, passed through a small device buffer)
+__global__ void combine_kernel(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ cf,
+                               double* __restrict__ y, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    double acc = 0.0;
+    for (int i = 0; i < m; ++i) acc += cf[i] * V[(int64_t)i * ldv + t];
+    y[t] = acc;
+  }
+}
+
+static unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, ceil_div(n, 256))); }
+
+struct GmresWork;
+static void gmres_free(GmresWork& W);
+
+struct GmresWork {
+  double* V = nullptr;      // (restart+1) x n
+  double* w = nullptr;
+  double* z = nullptr;
+  double* partial = nullptr;
+  double* coef = nullptr;   // device small buffer
+  double* hcoef = nullptr;  // pinned host buffer
+  int restart = 0;
+  int64_t n = 0;
+};
+
+static int gmres_alloc(pdes_ctx* c, GmresWork& W, int restart) {
+  if (c->gmres_ws) {
+    GmresWork* cached = (GmresWork*)c->gmres_ws;
+    if (cached->restart == restart && cached->n == c->numDof) { W = *cached; return PDES_OK; }
+    gmres_free(*cached);
+    delete cached;
+    c->gmres_ws = nullptr;
+  }
+  W.restart = restart;
+  W.n = c->numDof;
+  const size_t n = (size_t)c->numDof;
+  if (hipMalloc((void**)&W.V, (size_t)(restart + 1) * n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&W.w, n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&W.z, n * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&W.partial, (size_t)(restart + 2) * MD_BLOCKS * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&W.coef, (size_t)(2 * restart + 8) * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void**)&W.hcoef, (size_t)(2 * restart + 8) * sizeof(double)) != hipSuccess) {
+    set_error("GMRES workspace allocation failed");
+    return PDES_ERR_NOMEM;
+  }
+  return PDES_OK;
+}
+
+static void gmres_keep(pdes_ctx* c, GmresWork& W) {
+  if (!c->gmres_ws) c->gmres_ws = new GmresWork(W);
+}
+
+static void gmres_free(GmresWork& W) {
+  if (W.V) hipFree(W.V);
+  if (W.w) hipFree(W.w);
+  if (W.z) hipFree(W.z);
+  if (W.partial) hipFree(W.partial);
+  if (W.coef) hipFree(W.coef);
+  if (W.hcoef) hipHostFree(W.hcoef);
+  W = GmresWork();
+}
+
+void gmres_release(pdes_ctx* c) {
+  if (c->gmres_ws) {
+    gmres_free(*(GmresWork*)c->gmres_ws);
+    delete (GmresWork*)c->gmres_ws;
+    c->gmres_ws = nullptr;
+  }
+}
+
+// dots of w with V_0..V_{m-1} into W.coef[off..off+m)
+static int multidot(pdes_ctx* c, GmresWork& W, int m, const double* w, int off) {
+  const int nb = (int)std::min<int64_t>(MD_BLOCKS, std::max<int64_t>(1, ceil_div(W.n, MD_THREADS)));
+  hipLaunchKernelGGL(multidot_partial, dim3(nb), dim3(MD_THREADS), 0, c->stream, W.V, m, W.n, w, W.n, W.partial);
+  hipLaunchKernelGGL(multidot_reduce, dim3(m), dim3(256), 0, c->stream, W.partial, nb, W.coef + off);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+static int norm2_dev(pdes_ctx* c, GmresWork& W, const double* x, int off) {
+  return launch_wnorm2(c, x, nullptr, W.n, W.coef + off);
+}
+
+static int apply_op(pdes_ctx* c, int mode, const double* x, double* y) {
+  if (mode == 0) return launch_spmv(c, x, y, 1.0, nullptr, 0.0);
+  return launch_residual(c, c->d_q, x, nullptr, y, false);
+}
+
+// GMRES(restart), right preconditioned, x0 = x on entry (device vectors).
+int gmres_device(pdes_ctx* c, const double* b, double* x, double rtol, double atol, int restart, int maxit,
+                 int mode, int* iters, double* resnorm, bool x_zero) {
+  GmresWork W;
+  int st = gmres_alloc(c, W, restart);
+  if (st) return st;
+  const int64_t n = W.n;
+  std::vector<double> H((size_t)(restart + 1) * restart, 0.0), cs(restart), sn(restart), g(restart + 1);
+  int total = 0;
+  double beta = 0.0, beta0 = -1.0;
+  if (x_zero) PDES_HIP(hipMemsetAsync(x, 0, n * sizeof(double), c->stream));
+  bool converged = false;
+  while (true) {
+    // r = b - A x  -> V_0
+    if (x_zero && total == 0) {
+      hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, b, W.V, n);
+    } else {
+        if ((st = apply_op(c, mode, x, W.w)) != 0) goto done;
+      hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, b, W.V, n);
+      hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, W.w, W.V, -1.0, 1, n);
+    }
+    if ((st = norm2_dev(c, W, W.V, 0)) != 0) goto done;
+    PDES_HIP(hipMemcpyAsync(W.hcoef, W.coef, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));
+    beta = std::sqrt(W.hcoef[0]);
+    if (beta0 < 0) beta0 = beta;
+    if (beta <= std::max(rtol * beta0, atol) || total >= maxit) { converged = beta <= std::max(rtol * beta0, atol); break; }
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, W.V, W.V, 1.0 / beta, 0, n);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int jdone = 0;
+    double res_est = beta;
+    for (int j = 0; j < restart && total < maxit; ++j) {
+      double* Vj = W.V + (int64_t)j * n;
+      double* Vn = W.V + (int64_t)(j + 1) * n;
+      if ((st = pc_apply(c, Vj, W.z)) != 0) goto done;
+      if ((st = apply_op(c, mode, W.z, Vn)) != 0) goto done;
+      // CGS2: h = V^T w ; w -= V h ; twice
+      if ((st = multidot(c, W, j + 1, Vn, 0)) != 0) goto done;
+      hipLaunchKernelGGL(multiaxpy, dim3(grid_for(n)), dim3(256), 0, c->stream, W.V, j + 1, n, W.coef, Vn, n);
+      if ((st = multidot(c, W, j + 1, Vn, restart + 2)) != 0) goto done;
+      hipLaunchKernelGGL(multiaxpy, dim3(grid_for(n)), dim3(256), 0, c->stream, W.V, j + 1, n,
+                         W.coef + restart + 2, Vn, n);
+      if ((st = norm2_dev(c, W, Vn, 2 * restart + 4)) != 0) goto done;
+      PDES_HIP(hipMemcpyAsync(W.hcoef, W.coef, (2 * restart + 5) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      PDES_HIP(hipStreamSynchronize(c->stream));
+      double hn = std::sqrt(W.hcoef[2 * restart + 4]);
+      for (int i = 0; i <= j; ++i) H[(size_t)i * restart + j] = W.hcoef[i] + W.hcoef[restart + 2 + i];
+      H[(size_t)(j + 1) * restart + j] = hn;
+      if (hn > 0) hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, Vn, Vn, 1.0 / hn, 0, n);
+      // Givens
+      for (int i = 0; i < j; ++i) {
+        double a = H[(size_t)i * restart + j], bb = H[(size_t)(i + 1) * restart + j];
+        H[(size_t)i * restart + j] = cs[i] * a + sn[i] * bb;
+        H[(size_t)(i + 1) * restart + j] = -sn[i] * a + cs[i] * bb;
+      }
+      double a = H[(size_t)j * restart + j], bb = H[(size_t)(j + 1) * restart + j];
+      double r = std::hypot(a, bb);
+      cs[j] = r > 0 ? a / r : 1.0;
+      sn[j] = r > 0 ? bb / r : 0.0;
+      H[(size_t)j * restart + j] = r;
+      H[(size_t)(j + 1) * restart + j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      res_est = std::fabs(g[j + 1]);
+      ++total;
+      jdone = j + 1;
+      if (res_est <= std::max(rtol * beta0, atol) || hn == 0.0) break;
+    }
+    // y = H^-1 g ; x += M^-1 V y
+    {
+      std::vector<double> y(jdone);
+      for (int i = jdone - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int k = i + 1; k < jdone; ++k) s -= H[(size_t)i * restart + k] * y[k];
+        y[i] = s / H[(size_t)i * restart + i];
+      }
+      for (int i = 0; i < jdone; ++i) W.hcoef[i] = y[i];
+      PDES_HIP(hipMemcpyAsync(W.coef, W.hcoef, jdone * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(combine_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, W.V, jdone, n, W.coef, W.w, n);
+      if ((st = pc_apply(c, W.w, W.z)) != 0) goto done;
+      if (x_zero && total == jdone)
+        hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, W.z, x, 1.0, 0, n);
+      else
+        hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, W.z, x, 1.0, 1, n);
+      PDES_HIP(hipStreamSynchronize(c->stream));
+    }
+    x_zero = false;
+    if (res_est <= std::max(rtol * beta0, atol)) { converged = true; beta = res_est; break; }
+    if (total >= maxit) { beta = res_est; break; }
+  }
+done:
+  gmres_keep(c, W);
+  if (st) return st;
+  if (iters) *iters = total;
+  if (resnorm) *resnorm = beta;
+  (void)converged;
+  return PDES_OK;
+}
+
+}  // namespace pdes
+
+using namespace pdes;
+
+extern "C" {
+
+int pdes_pc_setup(pdes_ctx* c, int kind) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  int st = pc_setup(c, kind);
+  if (st) return st;
+  if (kind != PDES_PC_NONE) {
+    int32_t err = 0;
+    PDES_HIP(hipMemcpyAsync(&err, c->jac->d_err, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));
+    if (err) {
+      PDES_HIP(hipMemsetAsync(c->jac->d_err, 0, sizeof(int32_t), c->stream));
+      set_error("singular element block in the block-Jacobi factorisation");
+      return PDES_ERR_PHYS;
+    }
+  }
+  return PDES_OK;
+}
+
+int pdes_pc_apply(pdes_ctx* c, const double* b, double* x) {
+  PDES_CHECK(c && b && x, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(c->d_work, b, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = pc_apply(c, c->d_work, c->d_work2);
+  if (st) return st;
+  PDES_HIP(hipMemcpyAsync(x, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_gmres(pdes_ctx* c, const double* b_host, double* x_host, double rtol, double atol, int restart,
+               int maxit, int mode, int* iters, double* final_resnorm) {
+  PDES_CHECK(c && b_host && x_host && restart > 0 && maxit >= 0, PDES_ERR_ARG, "bad argument");
+  PDES_CHECK(mode == 1 || (c->jac && c->jac->assembled), PDES_ERR_ARG, "assemble the Jacobian for mode 0");
+  PDES_HIP(hipMemcpyAsync(c->d_work, b_host, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipMemcpyAsync(c->d_work2, x_host, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = gmres_device(c, c->d_work, c->d_work2, rtol, atol, restart, maxit, mode, iters, final_resnorm, false);
+  if (st) return st;
+  PDES_HIP(hipMemcpyAsync(x_host, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report* rep) {
+  PDES_CHECK(c && o && rep, PDES_ERR_ARG, "null argument");
+  auto t0 = std::chrono::steady_clock::now();
+  *rep = pdes_newton_report{};
+  const int64_t n = c->numDof;
+  int st;
+  double* res = c->d_res;
+  double* dq = nullptr;
+  double* rhs = nullptr;
+  PDES_CHECK(hipMalloc((void**)&dq, n * sizeof(double)) == hipSuccess &&
+             hipMalloc((void**)&rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "Newton workspace");
+  auto norm_res = [&](double* out) -> int {
+    int s2 = launch_residual(c, c->d_q, nullptr, res, nullptr, c->phys.check_positivity != 0);
+    if (s2) return s2;
+    s2 = launch_wnorm2(c, res, c->d_Minv, n, c->d_scalar);
+    if (s2) return s2;
+    double v = 0;
+    PDES_HIP(hipMemcpyAsync(&v, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));
+    s2 = sync_check_flag(c);
+    if (s2) return s2;
+    *out = std::sqrt(v);
+    return PDES_OK;
+  };
+  double nrm = 0;
+  if ((st = norm_res(&nrm)) != 0) goto fin;
+  rep->res_norm0 = nrm;
+  rep->res_norm = nrm;
+  if (nrm < o->res_abstol) { rep->converged = 1; goto fin; }
+  if ((st = jacobian_setup(c, o->fp64_blocks)) != 0) goto fin;
+  for (int it = 1; it <= o->itermax; ++it) {
+    // doRecalculation: Jacobian and preconditioner every iteration (recalc_freq 1)
+    if ((st = assemble(c)) != 0) goto fin;
+    if ((st = pdes_pc_setup(c, o->pc_kind)) != 0) goto fin;
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, res, rhs, -1.0, 0, n);
+    int kits = 0;
+    double kres = 0;
+    if ((st = gmres_device(c, rhs, dq, o->krylov_reltol, o->krylov_abstol, o->gmres_restart, o->krylov_itermax,
+                           o->jv_mode, &kits, &kres, true)) != 0)
+      goto fin;
+    rep->total_krylov_iters += kits;
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, dq, c->d_q, 1.0, 1, n);
+    // step norm: norm(delta_q_vec)
+    if ((st = launch_wnorm2(c, dq, nullptr, n, c->d_scalar + 1)) != 0) goto fin;
+    double sn2 = 0;
+    PDES_HIP(hipMemcpyAsync(&sn2, c->d_scalar + 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if ((st = norm_res(&nrm)) != 0) goto fin;
+    rep->step_norm = std::sqrt(sn2);
+    rep->res_norm = nrm;
+    rep->iterations = it;
+    // checkConvergence (newton.jl:402-455)
+    if (nrm < o->res_abstol || nrm / rep->res_norm0 < o->res_reltol || rep->step_norm <= o->step_tol) {
+      rep->converged = 1;
+      break;
+    }
+  }
+fin:
+  hipFree(dq);
+  hipFree(rhs);
+  rep->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+}  // extern "C"

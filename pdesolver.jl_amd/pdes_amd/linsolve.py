@@ -1,0 +1,215 @@
+"""Host-side mirror of the reference's Jacobian / linear-solver / Newton
+interfaces, backed by the gfx950 library.
+
+  evalJacobian            src/solver/euler/evalJacobian.jl:6-72 (explicit, BSR on the device)
+  getSparsityPattern      src/interface2.jl:544-555 / src/jacobian/sparse.jl:250-495
+  AbstractLO / AbstractPC src/linearsolvers/LinearSolvers.jl:185-645
+    HIPBSRLO        ~ PetscMatLO (assembled matrix)          lo_petscmat.jl
+    HIPMatFreeLO    ~ PetscMatFreeLO (complex-step products) newton_setup.jl:632-662
+    HIPBJacobiPC    ~ NewtonBDiagPC (element block Jacobi)   preconditioning_types.jl:41
+  StandardLinearSolver    LinearSolvers.jl:73-154 ; ls_standard.jl:28-491
+  newton                  src/NonlinearSolvers/newton.jl:54-304
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi
+from ._lib import check, lib
+from .euler import EulerData
+
+
+def evalJacobian(mesh, sbp, eqn: EulerData, opts=None, fp64_blocks: bool = False):
+    """Assemble the explicit Jacobian at the device state into the device BSR
+    matrix (block size ndof, FP32 blocks unless fp64_blocks)."""
+    check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))
+    check(lib().pdes_assemble_jacobian(eqn.ctx))
+
+
+def getSparsityPattern(eqn: EulerData, fp64_blocks: bool = False):
+    """(rowptr, colind) of the node-block BSR matrix (0-based, sorted)."""
+    check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))
+    nr = C.c_int64()
+    nz = C.c_int64()
+    check(lib().pdes_jacobian_nnzb(eqn.ctx, C.byref(nr), C.byref(nz)))
+    rowptr = np.zeros(nr.value + 1, dtype=np.int64)
+    colind = np.zeros(nz.value, dtype=np.int32)
+    check(lib().pdes_get_bsr_pattern(eqn.ctx, rowptr.ctypes.data, colind.ctypes.data))
+    return rowptr, colind
+
+
+def getJacobianValues(eqn: EulerData, fp64_blocks: bool = False):
+    nr = C.c_int64()
+    nz = C.c_int64()
+    check(lib().pdes_jacobian_nnzb(eqn.ctx, C.byref(nr), C.byref(nz)))
+    nd = eqn.mesh.numDofPerNode
+    vals = np.zeros((nz.value, nd, nd), dtype=np.float64 if fp64_blocks else np.float32)
+    check(lib().pdes_get_bsr_values(eqn.ctx, vals.ctypes.data))
+    return vals
+
+
+class AbstractLO:
+    def calcLinearOperator(self, mesh, sbp, eqn, opts=None, ctx_residual=None, t=0.0):
+        raise NotImplementedError
+
+    def applyLinearOperator(self, mesh, sbp, eqn, opts, ctx_residual, t, x, b):
+        raise NotImplementedError
+
+    def free(self):
+        pass
+
+
+class HIPBSRLO(AbstractLO):
+    """Assembled BSR Jacobian on the device; apply = BSR SpMV."""
+    jv_mode = 0
+
+    def __init__(self, eqn: EulerData, fp64_blocks: bool = False):
+        self.eqn = eqn
+        self.fp64_blocks = fp64_blocks
+
+    def calcLinearOperator(self, mesh=None, sbp=None, eqn=None, opts=None, ctx_residual=None, t=0.0):
+        evalJacobian(mesh, sbp, self.eqn, opts, self.fp64_blocks)
+
+    def applyLinearOperator(self, mesh, sbp, eqn, opts, ctx_residual, t, x, b):
+        xc = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty_like(xc)
+        check(lib().pdes_bsr_matvec(self.eqn.ctx, xc.ctypes.data, out.ctypes.data))
+        b[...] = out.reshape(b.shape)
+
+
+class HIPMatFreeLO(AbstractLO):
+    """Matrix-free operator: the directional derivative of the residual
+    (complex-step product of the reference) evaluated on the device."""
+    jv_mode = 1
+
+    def __init__(self, eqn: EulerData):
+        self.eqn = eqn
+
+    def calcLinearOperator(self, *a, **k):
+        pass
+
+    def applyLinearOperator(self, mesh, sbp, eqn, opts, ctx_residual, t, x, b):
+        xc = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.empty_like(xc)
+        check(lib().pdes_jv_complex_step(self.eqn.ctx, xc.ctypes.data, out.ctypes.data))
+        b[...] = out.reshape(b.shape)
+
+
+class AbstractPC:
+    def calcPC(self, mesh, sbp, eqn, opts=None, ctx_residual=None, t=0.0):
+        raise NotImplementedError
+
+    def applyPC(self, mesh, sbp, eqn, opts, t, b, x):
+        raise NotImplementedError
+
+    def free(self):
+        pass
+
+
+class PCNone(AbstractPC):
+    kind = abi.PDES_PC_NONE
+
+    def __init__(self, eqn: EulerData):
+        self.eqn = eqn
+
+    def calcPC(self, *a, **k):
+        check(lib().pdes_pc_setup(self.eqn.ctx, self.kind))
+
+    def applyPC(self, mesh, sbp, eqn, opts, t, b, x):
+        x[...] = b
+
+
+class HIPBJacobiPC(AbstractPC):
+    """Element block-Jacobi: LU of each element's (Np*ndof)^2 diagonal block,
+    FP32 factor and apply."""
+    kind = abi.PDES_PC_BJACOBI
+
+    def __init__(self, eqn: EulerData):
+        self.eqn = eqn
+
+    def calcPC(self, mesh=None, sbp=None, eqn=None, opts=None, ctx_residual=None, t=0.0):
+        check(lib().pdes_pc_setup(self.eqn.ctx, self.kind))
+
+    def applyPC(self, mesh, sbp, eqn, opts, t, b, x):
+        bc = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.empty_like(bc)
+        check(lib().pdes_pc_apply(self.eqn.ctx, bc.ctypes.data, out.ctypes.data))
+        x[...] = out.reshape(x.shape)
+
+
+class StandardLinearSolver:
+    """LinearSolver with a PC and an LO; linearSolve runs GMRES(restart)
+    with right preconditioning on the device."""
+
+    def __init__(self, pc: AbstractPC, lo: AbstractLO, reltol=1e-2, abstol=1e-50, itermax=1000, restart=30):
+        self.pc = pc
+        self.lo = lo
+        self.reltol = reltol
+        self.abstol = abstol
+        self.itermax = itermax
+        self.restart = restart
+        self.last_iters = 0
+        self.last_resnorm = 0.0
+
+    def setTolerances(self, reltol=-1, abstol=-1, dtol=-1, itermax=-1):
+        if reltol > 0:
+            self.reltol = reltol
+        if abstol > 0:
+            self.abstol = abstol
+        if itermax > 0:
+            self.itermax = itermax
+
+    def calcPCandLO(self, mesh=None, sbp=None, eqn=None, opts=None, ctx_residual=None, t=0.0):
+        self.lo.calcLinearOperator(mesh, sbp, eqn, opts, ctx_residual, t)
+        self.pc.calcPC(mesh, sbp, eqn, opts, ctx_residual, t)
+
+    def linearSolve(self, b, x, verbose=0):
+        eqn = self.lo.eqn
+        bc = np.ascontiguousarray(b, dtype=np.float64)
+        xc = np.ascontiguousarray(x, dtype=np.float64).copy()
+        its = C.c_int32()
+        rn = C.c_double()
+        check(lib().pdes_gmres(eqn.ctx, bc.ctypes.data, xc.ctypes.data, float(self.reltol), float(self.abstol),
+                               int(self.restart), int(self.itermax), int(self.lo.jv_mode), C.byref(its),
+                               C.byref(rn)))
+        x[...] = xc.reshape(x.shape)
+        self.last_iters = its.value
+        self.last_resnorm = rn.value
+        return its.value
+
+
+@dataclass
+class NewtonOptions:
+    """Option keys of the reference that fix the Newton solve
+    (src/input/read_input.jl:405-500)."""
+    res_abstol: float = 1e-10
+    res_reltol: float = 1e-300
+    step_tol: float = 0.0
+    itermax: int = 50
+    krylov_reltol: float = 1e-2
+    krylov_abstol: float = 1e-50
+    krylov_itermax: int = 1000
+    gmres_restart: int = 30
+    pc_kind: int = abi.PDES_PC_BJACOBI
+    fp64_blocks: bool = False
+    jv_mode: int = 0
+
+    def desc(self):
+        return abi.NewtonOpts(self.res_abstol, self.res_reltol, self.step_tol, self.itermax,
+                              self.krylov_reltol, self.krylov_abstol, self.krylov_itermax,
+                              self.gmres_restart, self.pc_kind, int(self.fp64_blocks), self.jv_mode)
+
+
+def newton(mesh, sbp, eqn: EulerData, opts: NewtonOptions | None = None):
+    """Steady Newton solve of R(q) = 0 on the device state (newtonInner):
+    explicit Jacobian + preconditioner recalculated every iteration, inexact
+    GMRES solve of J dq = -R, q += dq, until ||R||_{M^-1} < res_abstol.
+    Returns the report (iterations, converged, residual norms, time)."""
+    opts = opts or NewtonOptions()
+    o = opts.desc()
+    rep = abi.NewtonReport()
+    check(lib().pdes_newton_solve(eqn.ctx, C.byref(o), C.byref(rep)))
+    return rep

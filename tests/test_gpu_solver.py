@@ -1,0 +1,228 @@
+"""GPU parity of the explicit Jacobian (BSR), preconditioner, GMRES, Newton
+and functionals against the CPU oracle, through the C ABI.
+
+Reference tests restated: explicit Jacobian vs complex-step coloring Jacobian
+(test/euler/test_jac.jl:2284-2367), explicit matrix-vector product vs
+complex-step Jv (test_jac.jl:2370-2457), exact sparsity (test_jac.jl:2453-2460),
+Newton convergence to the absolute tolerance."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from pdes_amd import abi, euler, linsolve, problems
+from pdes_amd._lib import check, lib
+from helpers import setup, state_for
+
+pytestmark = pytest.mark.gpu
+
+JCASES = [("vortex", 3, 1, "omega", "roe"), ("vortex", 3, 2, "omega", "roe"), ("vortex", 2, 3, "omega", "roe"),
+          ("vortex", 3, 1, "diage", "roe"), ("vortex", 3, 2, "diage", "roe"), ("vortex", 2, 4, "diage", "roe"),
+          ("vortex", 3, 1, "diage", "ir"), ("vortex", 3, 2, "diage", "ir"), ("vortex", 2, 4, "diage", "ir"),
+          ("cube", 1, 1, "omega", "roe"), ("cube", 1, 2, "omega", "roe")]
+
+
+def _eqn(kind, n, p, opk, flux, q=None):
+    m, op, ph, b = setup(kind, n, p, opk, flux)
+    if q is None:
+        q = state_for(kind, m, ph)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    return m, op, ph, b, q, eqn
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", JCASES)
+def test_bsr_pattern_bit_exact(oracle, kind, n, p, opk, flux):
+    print(f"case {kind} n={n} p={p} {opk} {flux}", flush=True)
+    m, op, ph, b, q, eqn = _eqn(kind, n, p, opk, flux)
+    rowptr, colind = linsolve.getSparsityPattern(eqn)
+    r_ref, c_ref = oracle.sparsity_bsr(b)
+    assert np.array_equal(rowptr, r_ref)
+    assert np.array_equal(colind, c_ref)
+    eqn.free()
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", JCASES)
+def test_jacobian_fp64_matches_complex_step(oracle, kind, n, p, opk, flux):
+    print(f"case {kind} n={n} p={p} {opk} {flux}", flush=True)
+    m, op, ph, b, q, eqn = _eqn(kind, n, p, opk, flux)
+    linsolve.evalJacobian(m, op, eqn, fp64_blocks=True)
+    got = linsolve.getJacobianValues(eqn, fp64_blocks=True)
+    rowptr, colind = oracle.sparsity_bsr(b)
+    ref = oracle.jacobian_cs(b, q, rowptr, colind)
+    scale = np.abs(ref).max()
+    err = np.abs(got - ref).max()
+    assert err <= 1e-12 * scale, f"max err {err:.3e} scale {scale:.3e}"
+    # explicit matvec == complex-step Jv (test_jac.jl:2370-2457)
+    v = np.random.default_rng(2).standard_normal(q.shape)
+    jv_ref = oracle.jv_complex_step(b, q, v)
+    lo = linsolve.HIPBSRLO(eqn, fp64_blocks=True)
+    out = np.zeros_like(v)
+    lo.applyLinearOperator(m, op, eqn, None, None, 0.0, v, out)
+    assert np.abs(out - jv_ref).max() <= 1e-11 * np.abs(jv_ref).max()
+    eqn.free()
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", [JCASES[1], JCASES[4], JCASES[7], JCASES[10]])
+def test_jacobian_fp32_blocks(oracle, kind, n, p, opk, flux):
+    """FP32 storage of FP64-computed blocks: each entry is the rounded FP64 value."""
+    m, op, ph, b, q, eqn = _eqn(kind, n, p, opk, flux)
+    linsolve.evalJacobian(m, op, eqn, fp64_blocks=False)
+    got = linsolve.getJacobianValues(eqn, fp64_blocks=False)
+    assert got.dtype == np.float32
+    rowptr, colind = oracle.sparsity_bsr(b)
+    ref = oracle.jacobian_cs(b, q, rowptr, colind)
+    err = np.abs(got.astype(np.float64) - ref)
+    assert np.all(err <= 6e-8 * np.abs(ref) + 1e-12 * np.abs(ref).max())
+    eqn.free()
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", [JCASES[1], JCASES[5], JCASES[10]])
+def test_block_jacobi_apply(kind, n, p, opk, flux):
+    """NewtonBDiagPC: x = D_e^{-1} b per element, checked against numpy solves
+    of the same FP32 element blocks (FP32 factor/apply tolerance)."""
+    m, op, ph, b, q, eqn = _eqn(kind, n, p, opk, flux)
+    linsolve.evalJacobian(m, op, eqn, fp64_blocks=False)
+    vals = linsolve.getJacobianValues(eqn).astype(np.float64)
+    rowptr, colind = linsolve.getSparsityPattern(eqn)
+    pc = linsolve.HIPBJacobiPC(eqn)
+    pc.calcPC()
+    rhs = np.random.default_rng(3).standard_normal(q.shape)
+    x = np.zeros_like(rhs)
+    pc.applyPC(m, op, eqn, None, 0.0, rhs, x)
+    Np, nd = op.numnodes, m.dim + 2
+    nb = Np * nd
+    worst = 0.0
+    for e in range(m.numEl):
+        D = np.zeros((nb, nb))
+        for i in range(Np):
+            row = e * Np + i
+            for pos in range(rowptr[row], rowptr[row + 1]):
+                j = colind[pos] - e * Np
+                if 0 <= j < Np:
+                    D[i * nd:(i + 1) * nd, j * nd:(j + 1) * nd] = vals[pos]
+        xe = np.linalg.solve(D, rhs[e].ravel())
+        worst = max(worst, np.abs(x[e].ravel() - xe).max() / np.abs(xe).max())
+    assert worst < 1e-3, worst
+    eqn.free()
+
+
+@pytest.mark.parametrize("pc_kind", [abi.PDES_PC_NONE, abi.PDES_PC_BJACOBI])
+def test_gmres_solves_to_tolerance(pc_kind):
+    m, op, ph, b, q, eqn = _eqn("vortex", 4, 2, "diage", "roe")
+    lo = linsolve.HIPBSRLO(eqn, fp64_blocks=True)
+    pc = linsolve.HIPBJacobiPC(eqn) if pc_kind == abi.PDES_PC_BJACOBI else linsolve.PCNone(eqn)
+    # without a preconditioner restarted GMRES(30) stagnates on this DG
+    # Jacobian; a long restart keeps the unpreconditioned case convergent
+    restart = 30 if pc_kind == abi.PDES_PC_BJACOBI else 400
+    tol = 1e-10 if pc_kind == abi.PDES_PC_BJACOBI else 1e-8
+    ls = linsolve.StandardLinearSolver(pc, lo, reltol=tol, restart=restart, itermax=3000)
+    ls.calcPCandLO()
+    rhs = np.random.default_rng(4).standard_normal(q.shape)
+    x = np.zeros_like(rhs)
+    its = ls.linearSolve(rhs, x)
+    Ax = np.zeros_like(x)
+    lo.applyLinearOperator(m, op, eqn, None, None, 0.0, x, Ax)
+    rel = np.linalg.norm(Ax - rhs) / np.linalg.norm(rhs)
+    print("gmres iterations", its, "rel residual", rel)
+    assert its < 3000 and rel < 2 * tol
+    # the Givens estimate tracks the true residual
+    assert abs(ls.last_resnorm / np.linalg.norm(rhs) - rel) < 0.1 * rel + 1e-13
+    eqn.free()
+
+
+def _minv(m, op):
+    return np.repeat((1.0 / (op.w[None, :] / m.jac))[:, :, None], m.dim + 2, axis=2)
+
+
+def test_newton_config0_matches_direct_solve(oracle):
+    """Config 0: 20x20 quarter annulus (800 triangles), Omega p=1, Roe,
+    vortex BC, exact solution times (1 + U(-1e-3, 1e-3)).  The oracle runs
+    Newton with a sparse direct solve; the device runs Newton with FP64
+    blocks and GMRES to 1e-12: same iteration count within 1, both reach
+    ||R|| <= 1e-10 and the converged states agree."""
+    m, op, ph = problems.make_problem("vortex", 20, 1, "omega", "roe")
+    b = abi.make_descriptors(m, op, ph)
+    q0 = problems.vortex_noise_state(m, ph, seed=0)
+    q_ref, norms = oracle.newton_direct(b, q0, _minv(m, op), res_abstol=1e-10)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q0)
+    rep = linsolve.newton(m, op, eqn, linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-12,
+                                                             fp64_blocks=True, krylov_itermax=5000))
+    q = eqn.get_q()
+    print("oracle norms", norms, "device iterations", rep.iterations, "norm", rep.res_norm)
+    assert rep.converged and rep.res_norm <= 1e-10
+    assert abs(rep.iterations - (len(norms) - 1)) <= 1
+    assert np.abs(q - q_ref).max() <= 1e-9 * np.abs(q_ref).max()
+    eqn.free()
+
+
+def test_newton_mixed_precision_matches_fp64():
+    """FP32 Jacobian blocks and FP32 block-Jacobi under the FP64 inexact Newton
+    loop reach ||R|| <= 1e-10 and the same state as the FP64-everywhere run."""
+    m, op, ph = problems.make_problem("vortex", 12, 2, "diage", "roe")
+    q0 = problems.vortex_noise_state(m, ph, seed=0)
+    out = {}
+    for fp64 in (True, False):
+        eqn = euler.EulerData(m, op, ph)
+        eqn.set_q(q0)
+        rep = linsolve.newton(m, op, eqn, linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-6,
+                                                                 fp64_blocks=fp64))
+        assert rep.converged, (fp64, rep.res_norm)
+        out[fp64] = (eqn.get_q(), rep.iterations)
+        eqn.free()
+    dq = np.abs(out[True][0] - out[False][0]).max()
+    print("iterations fp64", out[True][1], "mixed", out[False][1], "max |dq|", dq)
+    assert dq <= 1e-10 * np.abs(out[True][0]).max()
+
+
+def test_newton_matrix_free():
+    """Matrix-free Jv (jv_mode 1) with the FP32 block-Jacobi preconditioner."""
+    m, op, ph = problems.make_problem("vortex", 8, 1, "omega", "roe")
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(problems.vortex_noise_state(m, ph, seed=0))
+    rep = linsolve.newton(m, op, eqn, linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-6, jv_mode=1))
+    assert rep.converged and rep.res_norm <= 1e-10
+    eqn.free()
+
+
+def test_newton_3d_cube():
+    m, op, ph = problems.make_problem("cube", 3, 2, "omega", "roe", Ma=0.5, aoa_deg=2.0)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(problems.freestream_bump_state(m, ph, seed=2))
+    rep = linsolve.newton(m, op, eqn, linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-4))
+    print("3D newton iterations", rep.iterations, "krylov", rep.total_krylov_iters, "norm", rep.res_norm)
+    assert rep.converged and rep.res_norm <= 1e-10
+    eqn.free()
+
+
+@pytest.mark.parametrize("kind,n,p,opk", [("vortex", 6, 2, "omega"), ("cube", 2, 2, "omega")])
+def test_functionals(oracle, kind, n, p, opk):
+    m, op, ph = problems.make_problem(kind, n, p, opk, "roe", Ma=0.5, aoa_deg=3.0)
+    q = problems.vortex_noise_state(m, ph) if kind == "vortex" else problems.freestream_bump_state(m, ph)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    bcs = list(range(m.numBC))
+    ref = oracle.functional(m, op, ph, q, bcs)
+    bc = np.asarray(bcs, dtype=np.int32)
+    val = np.zeros(3)
+    for kind_id, key in [(1, "lift"), (2, "drag"), (3, "entropy"), (4, "mass")]:
+        check(lib().pdes_eval_functional(eqn.ctx, kind_id, bc.ctypes.data, len(bcs), val.ctypes.data_as(C.POINTER(C.c_double))))
+        assert abs(val[0] - ref[key]) <= 1e-12 * (1 + abs(ref[key])), (key, val[0], ref[key])
+    eqn.free()
+
+
+def test_functionals_closed_form():
+    """Uniform freestream: zero net force and zero entropy flux over the
+    closed boundary (test/euler/test_functionals.jl:150-158)."""
+    m, op, ph = problems.make_problem("cube", 2, 2, "omega", "roe", Ma=0.5, aoa_deg=3.0)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(problems.freestream(m.coords.shape[:-1], 3, ph))
+    bc = np.arange(m.numBC, dtype=np.int32)
+    val = np.zeros(3)
+    check(lib().pdes_eval_functional(eqn.ctx, 0, bc.ctypes.data, len(bc), val.ctypes.data_as(C.POINTER(C.c_double))))
+    assert np.abs(val).max() < 1e-13
+    check(lib().pdes_eval_functional(eqn.ctx, 3, bc.ctypes.data, len(bc), val.ctypes.data_as(C.POINTER(C.c_double))))
+    assert abs(val[0]) < 1e-13
+    eqn.free()
