@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--steps", type=int, default=250)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 512 MiB device-to-device copy (torch) to calibrate HBM byte counters")
+    ap.add_argument("--variants", default="all", help="comma list like p2_roe,p4_ir (default all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X; "
@@ -105,11 +108,16 @@ def main():
     total_dof_evals = 0
     total_ms = 0.0
     kern = {}
+    sel = None if args.variants == "all" else set(args.variants.split(","))
     for p, flux in variants():
+        if sel is not None and f"p{p}_{flux}" not in sel:
+            continue
         m, op, ph, q = make(p, flux)
         eqn = euler.EulerData(m, op, ph)
         eqn.set_q(q)
         h = dt_for(m, q, ph, p)
+        tuned = euler.autotune(eqn, 10)
+        eqn.set_q(q)
         euler.rk4(eqn, args.warmup, h)
         eqn.set_q(q)
         ms = euler.rk4(eqn, args.steps, h, timed=True)
@@ -122,7 +130,7 @@ def main():
         kern[(p, flux)] = (kms, nbytes)
         results[f"p{p}_{flux}"] = {"dof_evals_per_s": dof_evals / (ms / 1e3), "ms_per_rk4_step": ms / args.steps,
                                    "residual_kernel_ms": kms, "residual_GBs": nbytes / (kms / 1e3) / 1e9,
-                                   "numDof": int(q.size)}
+                                   "numDof": int(q.size), "launch_variants": tuned}
         eqn.free()
     # dominant kernel by time in the step
     (pd, fd), (kms, nbytes) = max(kern.items(), key=lambda kv: kv[1][0])
@@ -147,6 +155,12 @@ def main():
         "roofline": {"kernel": f"euler_residual p={pd} {fd}", "bound": "hbm", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
     }
+    if args.calibrate_copy:
+        import torch
+        x = torch.empty(512 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

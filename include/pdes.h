@@ -128,6 +128,22 @@ int pdes_state_ptr(pdes_ctx* ctx, double** q_dev, double** res_dev);
  * check_positivity is set and a node has rho <= 0 or p <= 0. */
 int pdes_eval_residual(pdes_ctx* ctx, double t);
 
+/* Launch tuning (no counterpart in the reference): the residual kernel has
+ * NUM variants (elements per workgroup x waves per SIMD); pdes_autotune times
+ * each on this context's mesh, for the plain and the RK4-stage form, and keeps
+ * the fastest (chosen[0], chosen[1]).  pdes_set_launch_variant fixes them. */
+int pdes_autotune(pdes_ctx* ctx, int iters, int32_t* chosen);
+int pdes_set_launch_variant(pdes_ctx* ctx, int plain_variant, int rk_variant);
+
+/* Timing helpers: average residual-kernel duration over `iters` launches, and
+ * nsteps RK4 steps (rk4, src/NonlinearSolvers/rk4.jl:144-344) on the device
+ * state with 4 residual evaluations per step, h the time step; elapsed_ms (if
+ * not null) receives the HIP-event time of all steps. */
+int pdes_time_residual(pdes_ctx* ctx, int iters, double* avg_ms);
+int pdes_rk4(pdes_ctx* ctx, int nsteps, double h, double* elapsed_ms);
+int pdes_jv_device(pdes_ctx* ctx, const double* v_dev, double* b_dev);
+int pdes_sync(pdes_ctx* ctx);
+
 /* calcNorm(eqn, res, strongres=true)  src/Utils/Utils.jl:427-449 on the device
  * residual: sqrt(sum res_i^2 Minv_i). */
 int pdes_residual_norm(pdes_ctx* ctx, double* norm);

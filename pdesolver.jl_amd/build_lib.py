@@ -9,14 +9,14 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-OBJ = os.path.join(HERE, "_build")
-OUT = os.path.join(HERE, "pdes_amd", "libpdes_amd.so")
+OBJ = os.environ.get("PDES_BUILD_OBJ", os.path.join(HERE, "_build"))
+OUT = os.environ.get("PDES_BUILD_OUT", os.path.join(HERE, "pdes_amd", "libpdes_amd.so"))
 ARCH = os.environ.get("PDES_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-         "-Wno-unused-result", "-Wno-unused-parameter"]
+         "-Wno-unused-result", "-Wno-unused-parameter"] + os.environ.get("PDES_EXTRA_FLAGS", "").split()
 
 
 def _compile(src):

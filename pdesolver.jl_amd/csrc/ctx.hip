@@ -74,6 +74,9 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   c->pc.rho_free = ph->rho_free;
   c->pc.E_free = ph->p_free / (ph->gamma - 1) + 0.5 * ph->rho_free * ph->Ma * ph->Ma;  // types.jl:252
   c->pc.vortex_Min = ph->vortex_Min;
+  // default launch variants measured on MI355X: the weak form runs best at 3
+  // waves per SIMD, the register-heavier split form at 2
+  c->res_variant[0] = c->res_variant[1] = (ph->volume_integral_type == 2) ? 0 : 1;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     delete c;
@@ -156,7 +159,26 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   int st = PDES_OK;
   hipStream_t sm = c->stream;
   const size_t nI = m->numInterfaces, nB = m->numBoundaryFaces;
-  if (!st) st = upload(&c->d_dxidx, m->dxidx, (size_t)m->numEl * Np * dim * dim, sm);
+  // affine elements (the metric is bitwise identical at every node): keep one
+  // metric block per element in HBM instead of one per node
+  {
+    const size_t blk = (size_t)dim * dim;
+    bool affine = true;
+    for (int e = 0; e < m->numEl && affine; ++e)
+      for (int j = 1; j < Np && affine; ++j)
+        affine = std::memcmp(m->dxidx + ((size_t)e * Np + j) * blk, m->dxidx + (size_t)e * Np * blk,
+                             blk * sizeof(double)) == 0;
+    c->metric_per_node = affine ? 0 : 1;
+    if (affine) {
+      std::vector<double> compact((size_t)m->numEl * blk);
+      for (int e = 0; e < m->numEl; ++e)
+        std::memcpy(&compact[(size_t)e * blk], m->dxidx + (size_t)e * Np * blk, blk * sizeof(double));
+      if (!st) st = upload(&c->d_dxidx, compact.data(), compact.size(), sm);
+      if (!st) st = (hipStreamSynchronize(sm) == hipSuccess) ? PDES_OK : PDES_ERR_HIP;
+    } else {
+      if (!st) st = upload(&c->d_dxidx, m->dxidx, (size_t)m->numEl * Np * blk, sm);
+    }
+  }
   if (!st) st = upload(&c->d_elface, c->h_elface.data(), c->h_elface.size(), sm);
   if (!st) st = upload(&c->d_nrm_face, m->nrm_face, nI * Nf * dim, sm);
   if (!st) st = upload(&c->d_nrm_bndry, m->nrm_bndry, nB * Nf * dim, sm);

@@ -11,6 +11,7 @@ struct pdes_ctx {
   // problem shape
   int dim = 0, nd = 0, numEl = 0, Np = 0, Nf = 0, stencil = 0, norient = 0, nfaces = 0;
   int diag_e = 0, numIface = 0, numBndry = 0, numBC = 0;
+  int metric_per_node = 1;    // 0 when every element's dxidx is the same at all its nodes
   int64_t numDof = 0;
   pdes_phys_desc phys{};
   pdes::PhysConst pc{};
@@ -42,15 +43,16 @@ struct pdes_ctx {
   double* d_scalar = nullptr;   // reduction results
   int32_t* d_flag = nullptr;    // positivity flag
   int partial_n = 0;
-  // launch configuration (elements per block), chosen per operator
-  int epb_weak = 0;
+  // residual launch variants (see residual_launch.hpp) for the plain and the
+  // RK-epilogue kernel, chosen by the autotuner or the default table
+  int res_variant[2] = {1, 1};
   // solver state
   pdes_jacobian* jac = nullptr;
   pdes_precond* pc_obj = nullptr;
   void* gmres_ws = nullptr;     // cached GMRES workspace
 
   pdes::DevMesh devmesh() const {
-    return pdes::DevMesh{numEl, d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};
+    return pdes::DevMesh{numEl, metric_per_node, d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};
   }
   pdes::DevSBP devsbp() const {
     return pdes::DevSBP{d_Q, d_interp, d_perm, d_nbrperm, d_nodepos, d_wface, norient};
@@ -60,7 +62,7 @@ struct pdes_ctx {
 namespace pdes {
 // residual launch (value run, or directional derivative when v != nullptr)
 int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
-                    bool check);
+                    bool check, const RKStage* rk = nullptr);
 // deterministic sum of x_i^2 * w_i (w may be null) into *out_dev
 int launch_wnorm2(pdes_ctx* c, const double* x, const double* w, int64_t n, double* out_dev);
 int sync_check_flag(pdes_ctx* c);

@@ -141,16 +141,18 @@ __device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T
   for (int i = 0; i < ND; ++i) flux[i] = sat[i] + ef[i];
 }
 
-// logavg  bc_solvers.jl:942-956
+// logavg  bc_solvers.jl:942-956.  f = (xi-1)/(xi+1) with xi = aL/aR is
+// formed as (aL-aR)/(aL+aR) (one division instead of two; same value to
+// rounding), and xi is only formed on the log branch.
 template <class T>
 __device__ __forceinline__ T logavg(const T& aL, const T& aR) {
-  T xi = aL / aR;
-  T f = (xi - 1.0) / (xi + 1.0);
+  const T s = aL + aR;
+  T f = (aL - aR) / s;
   T u = f * f;
   T F;
   if (re(u) < 1e-3) F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
-  else F = (dlog(xi) / 2.0) / f;
-  return (aL + aR) / (2.0 * F);
+  else F = dlog(aL / aR) / (2.0 * f);
+  return s / (2.0 * F);
 }
 
 // Ismail-Roe parameter vector of one state: z1 = sqrt(rho/p), z_{1+d} = z1 u_d,
@@ -174,16 +176,16 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
   T la = logavg(zL[ND - 1], zR[ND - 1]);
   T zs = zL[0] + zR[0];
+  T rzs = 1.0 / zs;
   T rho_hat = 0.5 * zs * la;
   T vh[3];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) vh[d] = (zL[1 + d] + zR[1 + d]) / zs;
-  T p1_hat = (zL[ND - 1] + zR[ND - 1]) / zs;
-  T p2_hat = ((gamma + 1) / (2 * gamma)) * la / logavg(zL[0], zR[0]) +
-             (gamma_1 / (2 * gamma)) * (zL[ND - 1] + zR[ND - 1]) / zs;
+  for (int d = 0; d < DIM; ++d) vh[d] = (zL[1 + d] + zR[1 + d]) * rzs;
+  T p1_hat = (zL[ND - 1] + zR[ND - 1]) * rzs;
+  T p2_hat = ((gamma + 1) / (2 * gamma)) * la / logavg(zL[0], zR[0]) + (gamma_1 / (2 * gamma)) * p1_hat;
   T ke = vh[0] * vh[0] + vh[1] * vh[1];
   if constexpr (DIM == 3) ke = ke + vh[2] * vh[2];
-  T h_hat = gamma * p2_hat / (rho_hat * gamma_1) + 0.5 * ke;
+  T h_hat = (gamma / gamma_1) * p2_hat / rho_hat + 0.5 * ke;
 #pragma unroll
   for (int i = 0; i < NDIR; ++i) {
     T un = dir[0 * NDIR + i] * vh[0] + dir[1 * NDIR + i] * vh[1];

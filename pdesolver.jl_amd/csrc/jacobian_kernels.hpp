@@ -102,7 +102,7 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
       for (int c = 0; c < ND; ++c) { qd[c] = dual<ND>(sm.q[j][c]); qd[c].d[c] = 1.0; }
       double nrm[DIM];
 #pragma unroll
-      for (int x = 0; x < DIM; ++x) nrm[x] = mesh.dxidx[((e * NP + j) * DIM + x) * DIM + d];
+      for (int x = 0; x < DIM; ++x) nrm[x] = metric<DIM, NP>(mesh, e, j)[x * DIM + d];
       dual<ND> F[ND];
       euler_flux<DIM>(phys, qd, nrm, F);
 #pragma unroll
@@ -216,7 +216,7 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
 #pragma unroll
           for (int x = 0; x < DIM; ++x)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = mesh.dxidx[((e * NP + hi) * DIM + x) * DIM + d];
+            for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, hi)[x * DIM + d];
           ir_flux_z<DIM, DIM>(phys, za, zb, nrmD, Fd);
           // derivative w.r.t. q_j: directions of hi or lo
           const int off = (j == hi) ? 0 : ND;

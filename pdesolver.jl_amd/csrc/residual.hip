@@ -1,49 +1,18 @@
 // residual.hip -- operator-shape dispatch of the fused residual kernel and the
 // reduction / vector kernels used around it.
-#include "ctx.hpp"
+#include "residual_launch.hpp"
 #include "shapes.hpp"
 
 namespace pdes {
 
-template <class Op, class T>
-constexpr int epb_for() {
-  // elements per block: fill up to 256 threads (128 for derivative runs)
-  constexpr int maxthreads = std::is_same<T, double>::value ? 256 : 128;
-  return maxthreads / Op::NP;
-}
-
-template <class Op, int VOL, int FLUX, class T>
-static int launch_one(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
-                      int32_t* flag) {
-  constexpr int EPB = epb_for<Op, T>();
-  const int threads = EPB * Op::NP;
-  const int blocks = (int)ceil_div(c->numEl, EPB);
-  hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T>), dim3(blocks), dim3(threads), 0,
-                     c->stream, c->devmesh(), c->devsbp(), c->pc, q, v, res, jv, flag);
-  PDES_HIP(hipGetLastError());
-  return PDES_OK;
-}
-
-template <class Op, class T>
-static int launch_shape(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
-                        int32_t* flag) {
-  const int vol = c->phys.volume_integral_type, flux = c->phys.face_flux;
-  if (vol == 1 && flux == PDES_FLUX_ROE) return launch_one<Op, VOL_WEAK, PDES_FLUX_ROE, T>(c, q, v, res, jv, flag);
-  if constexpr (Op::DIAGE && Op::DIM == 2) {
-    if (vol == 2 && flux == PDES_FLUX_IRSLF)
-      return launch_one<Op, VOL_SPLIT, PDES_FLUX_IRSLF, T>(c, q, v, res, jv, flag);
-  }
-  set_error("unsupported combination volume_integral_type=%d face_flux=%d for this operator", vol, flux);
-  return PDES_ERR_UNSUPPORTED;
-}
-
-int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, bool check) {
+int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, bool check,
+                    const RKStage* rk) {
   int32_t* flag = check ? c->d_flag : nullptr;
 #define PDES_DISPATCH(D, NP, NF, ST, DE)                                                        \
   if (c->dim == D && c->Np == NP && c->Nf == NF && c->stencil == ST && (c->diag_e != 0) == DE) { \
     using Op = OpShape<D, NP, NF, ST, DE>;                                                      \
-    if (v) return launch_shape<Op, dual<1>>(c, q, v, res, jv, flag);                            \
-    return launch_shape<Op, double>(c, q, v, res, jv, flag);                                    \
+    if (v) return launch_shape<Op, dual<1>>(c, q, v, res, jv, flag, nullptr, 0);                \
+    return launch_shape<Op, double>(c, q, v, res, jv, flag, rk, c->res_variant[rk ? 1 : 0]);    \
   }
   PDES_FOR_EACH_SHAPE(PDES_DISPATCH)
 #undef PDES_DISPATCH
@@ -99,27 +68,6 @@ int launch_wnorm2(pdes_ctx* c, const double* x, const double* w, int64_t n, doub
   return PDES_OK;
 }
 
-// RK4 stage update (rk4.jl:238-319 with pde_post_func :446-460, Minv scaling):
-//   k = Minv .* res ; acc += wacc * k ; q_next = q_old + cstage * k
-__global__ void rk4_stage_update(const double* __restrict__ res, const double* __restrict__ Minv,
-                                 const double* __restrict__ q_old, double* __restrict__ acc,
-                                 double* __restrict__ q_next, double wacc, double cstage, int first,
-                                 int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    double k = res[i] * Minv[i];
-    double a = first ? wacc * k : acc[i] + wacc * k;
-    acc[i] = a;
-    q_next[i] = q_old[i] + cstage * k;
-  }
-}
-
-__global__ void rk4_finish(double* __restrict__ q_old, const double* __restrict__ acc, double h6, int64_t n) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    q_old[i] = q_old[i] + h6 * acc[i];
-}
-
 }  // namespace pdes
 
 using namespace pdes;
@@ -161,6 +109,50 @@ int pdes_jv_device(pdes_ctx* c, const double* v_dev, double* b_dev) {
   return launch_residual(c, c->d_q, v_dev, nullptr, b_dev, false);
 }
 
+// Autotuner: times every launch variant of the residual kernel (plain and
+// RK-epilogue forms) on this context's mesh and keeps the fastest.
+int pdes_autotune(pdes_ctx* c, int iters, int32_t* chosen) {
+  PDES_CHECK(c && iters > 0, PDES_ERR_ARG, "bad argument");
+  hipEvent_t a, b;
+  PDES_HIP(hipEventCreate(&a));
+  PDES_HIP(hipEventCreate(&b));
+  for (int rkmode = 0; rkmode < 2; ++rkmode) {
+    float best = 1e30f;
+    int bestv = c->res_variant[rkmode];
+    for (int var = 0; var < NUM_RES_VARIANTS; ++var) {
+      c->res_variant[rkmode] = var;
+      // stage-1 epilogue: reads q and the accumulator, writes scratch buffers only
+      RKStage rk{c->d_Minv, c->d_q, c->d_work, c->d_work2, 2.0, 0.0, 0.0, 1};
+      const RKStage* rkp = rkmode ? &rk : nullptr;
+      int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);  // warm-up
+      if (st) return st;
+      PDES_HIP(hipEventRecord(a, c->stream));
+      for (int it = 0; it < iters; ++it) {
+        st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);
+        if (st) return st;
+      }
+      PDES_HIP(hipEventRecord(b, c->stream));
+      PDES_HIP(hipEventSynchronize(b));
+      float ms = 0;
+      PDES_HIP(hipEventElapsedTime(&ms, a, b));
+      if (ms < best) { best = ms; bestv = var; }
+    }
+    c->res_variant[rkmode] = bestv;
+    if (chosen) chosen[rkmode] = bestv;
+  }
+  PDES_HIP(hipEventDestroy(a));
+  PDES_HIP(hipEventDestroy(b));
+  return PDES_OK;
+}
+
+int pdes_set_launch_variant(pdes_ctx* c, int plain_variant, int rk_variant) {
+  PDES_CHECK(c && plain_variant >= 0 && plain_variant < NUM_RES_VARIANTS && rk_variant >= 0 &&
+             rk_variant < NUM_RES_VARIANTS, PDES_ERR_ARG, "launch variant out of range");
+  c->res_variant[0] = plain_variant;
+  c->res_variant[1] = rk_variant;
+  return PDES_OK;
+}
+
 // Times `iters` back-to-back residual evaluations on the context stream with
 // HIP events; *avg_ms is the average per evaluation (one kernel launch each).
 int pdes_time_residual(pdes_ctx* c, int iters, double* avg_ms) {
@@ -189,12 +181,10 @@ int pdes_time_residual(pdes_ctx* c, int iters, double* avg_ms) {
 // bracketed by events: *elapsed_ms gets the wall time of all steps.
 int pdes_rk4(pdes_ctx* c, int nsteps, double h, double* elapsed_ms) {
   PDES_CHECK(c && nsteps >= 0, PDES_ERR_ARG, "bad argument");
-  const int64_t n = c->numDof;
-  const int threads = 256;
-  const int blocks = (int)std::min<int64_t>(8192, ceil_div(n, threads));
   double* q_old = c->d_q;
   double* acc = c->d_work;
-  double* q_stage = c->d_work2;
+  double* stage_a = c->d_work2;
+  double* stage_b = c->d_res;   // residual storage doubles as the second stage buffer
   hipEvent_t a = nullptr, b = nullptr;
   if (elapsed_ms) {
     PDES_HIP(hipEventCreate(&a));
@@ -203,15 +193,15 @@ int pdes_rk4(pdes_ctx* c, int nsteps, double h, double* elapsed_ms) {
   }
   const double cst[4] = {h / 2, h / 2, h, 0.0};
   const double wst[4] = {1.0, 2.0, 2.0, 1.0};
+  // stage inputs/outputs: q_old -> A -> B -> A -> q_old (in place)
+  const double* qin[4] = {q_old, stage_a, stage_b, stage_a};
+  double* qout[4] = {stage_a, stage_b, stage_a, q_old};
   for (int s = 0; s < nsteps; ++s) {
     for (int stage = 0; stage < 4; ++stage) {
-      const double* qin = stage == 0 ? q_old : q_stage;
-      int st = launch_residual(c, qin, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+      RKStage rk{c->d_Minv, q_old, acc, qout[stage], wst[stage], cst[stage], h / 6, stage};
+      int st = launch_residual(c, qin[stage], nullptr, nullptr, nullptr, c->phys.check_positivity != 0, &rk);
       if (st) return st;
-      hipLaunchKernelGGL(rk4_stage_update, dim3(blocks), dim3(threads), 0, c->stream, c->d_res, c->d_Minv,
-                         q_old, acc, q_stage, wst[stage], cst[stage], stage == 0 ? 1 : 0, n);
     }
-    hipLaunchKernelGGL(rk4_finish, dim3(blocks), dim3(threads), 0, c->stream, q_old, acc, h / 6, n);
   }
   PDES_HIP(hipGetLastError());
   if (elapsed_ms) {

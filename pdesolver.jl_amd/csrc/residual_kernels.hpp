@@ -29,7 +29,8 @@ struct ElFace { int32_t x, y, z, w; };
 
 struct DevMesh {
   int32_t numEl;
-  const double* __restrict__ dxidx;       // [e][j][x][xi]
+  int32_t metric_per_node;                // 1: dxidx[e][j][x][xi]; 0: affine, dxidx[e][x][xi]
+  const double* __restrict__ dxidx;
   const ElFace* __restrict__ elface;      // [e][f]
   const double* __restrict__ nrm_face;    // [iface][k][dim]
   const double* __restrict__ nrm_bndry;   // [b][k][dim]
@@ -45,6 +46,12 @@ struct DevSBP {
   const double* __restrict__ wface;    // [k]
   int32_t norient;
 };
+
+// the DIM x DIM metric block [x][xi] of node j of element e
+template <int DIM, int NP>
+__device__ __forceinline__ const double* metric(const DevMesh& m, int64_t e, int j) {
+  return m.dxidx + (m.metric_per_node ? (e * NP + j) : e) * (DIM * DIM);
+}
 
 template <int DIM_, int NP_, int NF_, int ST_, bool DIAGE_>
 struct OpShape {
@@ -71,6 +78,18 @@ template <> struct Io<dual<1>> {
 
 enum { VOL_WEAK = 1, VOL_SPLIT = 2 };
 
+// RK4 stage epilogue (rk4.jl:238-319 with pde_post_func :446-460): instead of
+// storing res, k = Minv .* res updates the stage accumulator and writes the
+// next stage state (or, in the last stage, the new solution in place).
+struct RKStage {
+  const double* Minv;
+  const double* q_old;           // aliases q_next in the last stage (in-place update)
+  double* acc;
+  double* q_next;                // stage state written (stages 1-3) / q_old updated (stage 4)
+  double wacc, cstage, h6;
+  int32_t stage;                 // 0..3
+};
+
 template <class Op, int VOL, int FLUX, int EPB, class T>
 struct ResidualSmem {
   static constexpr int NP = Op::NP, ND = Op::ND, DIM = Op::DIM, NF = Op::NF, NFACES = Op::NFACES;
@@ -86,11 +105,12 @@ struct ResidualSmem {
   T G[EPB][NFACES][NF][ND];
 };
 
-template <class Op, int VOL, int FLUX, int EPB, class T>
-__global__ void __launch_bounds__(256)
+template <class Op, int VOL, int FLUX, int EPB, class T, bool RK = false, int WAVES = 2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
                       const double* __restrict__ qin, const double* __restrict__ vin,
-                      double* __restrict__ res, double* __restrict__ jv, int32_t* __restrict__ flag) {
+                      double* __restrict__ res, double* __restrict__ jv, int32_t* __restrict__ flag,
+                      RKStage rk) {
   constexpr int DIM = Op::DIM, NP = Op::NP, NF = Op::NF, ST = Op::ST, ND = Op::ND, NFACES = Op::NFACES;
   using Smem = ResidualSmem<Op, VOL, FLUX, EPB, T>;
   __shared__ Smem sm;
@@ -125,7 +145,33 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
     for (int x = 0; x < DIM; ++x)
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) dxi[x][d] = mesh.dxidx[((e * NP + i) * DIM + x) * DIM + d];
+      for (int d = 0; d < DIM; ++d) dxi[x][d] = metric<DIM, NP>(mesh, e, i)[x * DIM + d];
+  }
+
+  // diagonal-E: the record, normal and neighbour face-node state of this
+  // thread's first face slot are loaded before the volume phase, so the
+  // dependent global loads (record -> neighbour) overlap with it.
+  ElFace rec0{-1, 0, 0, 0};
+  T qn0[ND];
+  double nrm0[DIM];
+  if constexpr (Op::DIAGE) {
+    if (active && i < NFACES * NF) {
+      const int f = i / NF, k = i - f * NF;
+      rec0 = mesh.elface[e * NFACES + f];
+      const double* nr;
+      if (rec0.x >= 0) {
+        const int fN = rec0.z & 0xff, o = (rec0.z >> 8) & 0xff, side = (rec0.z >> 16) & 1;
+        const int kR = sbp.nbrperm[k * sbp.norient + o];
+        const int nodeN = sbp.perm[(side == 0 ? kR : k) * NFACES + fN];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) qn0[c] = Io<T>::load(qin, vin, ((int64_t)rec0.x * NP + nodeN) * ND + c);
+        nr = mesh.nrm_face + ((int64_t)rec0.y * NF + k) * DIM;
+      } else {
+        nr = mesh.nrm_bndry + ((int64_t)rec0.y * NF + k) * DIM;
+      }
+#pragma unroll
+      for (int x = 0; x < DIM; ++x) nrm0[x] = nr[x];
+    }
   }
 
   T ri[ND];
@@ -195,7 +241,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
           for (int x = 0; x < DIM; ++x)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = mesh.dxidx[((e * NP + jnode) * DIM + x) * DIM + d];
+            for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, jnode)[x * DIM + d];
           T Fd[DIM * ND];
           if (i_is_j) ir_flux_z<DIM, DIM>(phys, zi, zk, nrmD, Fd);
           else ir_flux_z<DIM, DIM>(phys, zk, zi, nrmD, Fd);
@@ -225,6 +271,83 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   }
 
   // ---- faces: slot (f, k) with k in elementL's face-node numbering
+  if constexpr (Op::DIAGE) {
+    // SparseFace: R selects face nodes, so each slot needs one own node and
+    // one neighbour node
+    if (active) {
+      for (int slot = i; slot < NFACES * NF; slot += NP) {
+        const int f = slot / NF;
+        const int k = slot - f * NF;
+        ElFace rec;
+        T qn[ND];
+        double nrm[DIM];
+        if (slot == i) {
+          rec = rec0;
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qn[c] = qn0[c];
+#pragma unroll
+          for (int x = 0; x < DIM; ++x) nrm[x] = nrm0[x];
+        } else {
+          rec = mesh.elface[e * NFACES + f];
+          const double* nr;
+          if (rec.x >= 0) {
+            const int fN = rec.z & 0xff, o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
+            const int kR = sbp.nbrperm[k * sbp.norient + o];
+            const int nodeN = sm.perm[side == 0 ? kR : k][fN];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) qn[c] = Io<T>::load(qin, vin, ((int64_t)rec.x * NP + nodeN) * ND + c);
+            nr = mesh.nrm_face + ((int64_t)rec.y * NF + k) * DIM;
+          } else {
+            nr = mesh.nrm_bndry + ((int64_t)rec.y * NF + k) * DIM;
+          }
+#pragma unroll
+          for (int x = 0; x < DIM; ++x) nrm[x] = nr[x];
+        }
+        T qL[ND], qR[ND], Fk[ND];
+        int m;
+        double sign;
+        if (rec.x < 0) {
+          m = k;
+          sign = -1.0;
+          const int node = sm.perm[k][f];
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qL[c] = sm.q[el][node][c];
+          double qg[ND];
+          if constexpr (DIM == 2) {
+            if (rec.w == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(phys, mesh.coords_bndry + ((int64_t)rec.y * NF + k) * DIM, qg);
+            else free_stream<DIM>(phys, qg);
+          } else {
+            free_stream<DIM>(phys, qg);
+          }
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qR[c] = qg[c];
+          roe_flux<DIM>(phys, qL, qR, nrm, Fk);
+        } else {
+          const int o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
+          const int kR = sbp.nbrperm[k * sbp.norient + o];
+          if (side == 0) {
+            m = k;
+            sign = -1.0;
+            const int node = sm.perm[k][f];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) { qL[c] = sm.q[el][node][c]; qR[c] = qn[c]; }
+          } else {
+            m = kR;
+            sign = 1.0;
+            const int node = sm.perm[kR][f];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) { qL[c] = qn[c]; qR[c] = sm.q[el][node][c]; }
+          }
+          if constexpr (FLUX == PDES_FLUX_ROE) roe_flux<DIM>(phys, qL, qR, nrm, Fk);
+          else irslf_flux<DIM>(phys, qL, qR, nrm, Fk);
+        }
+        const double wgt = sign * sm.wface[m];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = wgt * Fk[c];
+      }
+    }
+  } else
+  {
   if (active) {
     for (int slot = i; slot < NFACES * NF; slot += NP) {
       const int f = slot / NF;
@@ -326,6 +449,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = wgt * Fk[c];
     }
   }
+  }
   __syncthreads();
   if (active) {
     // res_i += sum_f sum_m interp[s(i,f)][m] G[f][m]
@@ -345,8 +469,25 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         }
       }
     }
+    if constexpr (RK) {
+      const int64_t base = (e * NP + i) * ND;
+      const double minv = rk.Minv[base];
 #pragma unroll
-    for (int c = 0; c < ND; ++c) Io<T>::store(res, jv, (e * NP + i) * ND + c, ri[c]);
+      for (int c = 0; c < ND; ++c) {
+        const double k = re(ri[c]) * minv;
+        const double qo = rk.stage == 0 ? re(qi[c]) : rk.q_old[base + c];
+        if (rk.stage == 3) {
+          rk.q_next[base + c] = qo + rk.h6 * (rk.acc[base + c] + rk.wacc * k);
+        } else {
+          const double a = rk.stage == 0 ? rk.wacc * k : rk.acc[base + c] + rk.wacc * k;
+          rk.acc[base + c] = a;
+          rk.q_next[base + c] = qo + rk.cstage * k;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < ND; ++c) Io<T>::store(res, jv, (e * NP + i) * ND + c, ri[c]);
+    }
   }
 }
 

@@ -10,7 +10,7 @@ import os
 from . import abi
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpdes_amd.so")
+LIB_PATH = os.environ.get("PDES_AMD_LIB", os.path.join(_HERE, "libpdes_amd.so"))
 _lib = None
 
 _vp = C.c_void_p
@@ -35,6 +35,8 @@ _SIGS = {
     "pdes_jv_complex_step": [_vp, _vp, _vp],
     "pdes_jv_device": [_vp, _vp, _vp],
     "pdes_time_residual": [_vp, C.c_int, _dp],
+    "pdes_autotune": [_vp, C.c_int, _i32p],
+    "pdes_set_launch_variant": [_vp, C.c_int, C.c_int],
     "pdes_rk4": [_vp, C.c_int, C.c_double, _dp],
     "pdes_jacobian_setup": [_vp, C.c_int],
     "pdes_jacobian_nnzb": [_vp, _i64p, _i64p],

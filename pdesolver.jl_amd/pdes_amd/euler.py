@@ -122,6 +122,14 @@ def rk4(eqn: EulerData, nsteps: int, h: float, timed: bool = False):
     return ms.value if timed else None
 
 
+def autotune(eqn: EulerData, iters: int = 10):
+    """Pick the fastest residual launch variant for this mesh/operator
+    (plain and RK4-stage kernels); returns the chosen variant indices."""
+    out = (C.c_int32 * 2)()
+    check(lib().pdes_autotune(eqn.ctx, int(iters), out))
+    return int(out[0]), int(out[1])
+
+
 def time_residual(eqn: EulerData, iters: int) -> float:
     """Average duration (ms) of the residual kernel over `iters` launches."""
     ms = C.c_double(0.0)

@@ -120,3 +120,22 @@ def test_rk4_matches_oracle(oracle):
         q = q + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
     assert np.abs(got - q).max() < 1e-12 * np.abs(q).max()
     eqn.free()
+
+
+def test_every_launch_variant_parity(oracle):
+    """The autotuner's launch variants (elements per workgroup x waves per
+    SIMD) all give the oracle's residual; autotune picks a valid one."""
+    from pdes_amd._lib import check, lib
+    m, op, ph, b = setup("vortex", 6, 2, "diage", "ir")
+    q = state_for("vortex", m, ph)
+    ref = oracle.residual(b, q)
+    scale = oracle.residual_scale(b, q)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    for var in range(4):
+        check(lib().pdes_set_launch_variant(eqn.ctx, var, var))
+        euler.evalResidual(m, op, eqn)
+        assert_parity(eqn.res, ref, scale, 1e-12, f"variant {var}")
+    chosen = euler.autotune(eqn, 2)
+    assert all(0 <= v < 4 for v in chosen)
+    eqn.free()
