@@ -184,7 +184,13 @@ int pdes_gmres(pdes_ctx* ctx, const double* b_host, double* x_host, double rtol,
                double* final_resnorm);
 
 /* Newton (newtonInner, src/NonlinearSolvers/newton.jl:137-304) on the device
- * state: J dq = -R, q += dq until ||R||_{M^-1} <= res_abstol or reltol. */
+ * state: J dq = -R, q += dq until ||R||_{M^-1} <= res_abstol or reltol.
+ * With globalize_euler (opts "newton_globalize_euler", implicit-Euler
+ * pseudo-time stepping of src/NonlinearSolvers/globalization.jl:181-350) the
+ * operator is J - M/tau_vec, tau_vec[node] = tau/(1 + sqrt(jac[node])),
+ * tau = euler_tau at the first Jacobian and tau *= ||R_prev||/||R|| at each
+ * later one; scale_euler ("newton_scale_euler") makes the first update use
+ * ||R_prev|| = 1. */
 typedef struct {
   double res_abstol;
   double res_reltol;
@@ -197,6 +203,9 @@ typedef struct {
   int32_t pc_kind;             /* PDES_PC_* */
   int32_t fp64_blocks;         /* store Jacobian/PC in FP64 instead of FP32 */
   int32_t jv_mode;             /* 0 BSR, 1 matrix-free complex step */
+  int32_t globalize_euler;     /* implicit-Euler globalization on/off */
+  double euler_tau;            /* initial pseudo-time step (1.0) */
+  int32_t scale_euler;         /* newton_scale_euler */
 } pdes_newton_opts;
 
 typedef struct {
@@ -218,6 +227,27 @@ int pdes_newton_solve(pdes_ctx* ctx, const pdes_newton_opts* opts,
  * 4 = mass flow; over the BC groups listed in bcnums. */
 int pdes_eval_functional(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
                          double* val);
+
+/* Batched steady solves (no single counterpart in the reference, which runs
+ * one solve per process): ninst independent problems, each described as for
+ * pdes_ctx_create.  Instances that share an SBP operator and physics kind are
+ * packed into one group whose kernels cover all its unconverged instances in
+ * one launch; reductions, Krylov scalars and convergence tests are per
+ * instance, so a result does not depend on the other instances in the batch.
+ * Groups run concurrently on separate streams.  reports[i].converged is 1 on
+ * convergence, 0 if itermax was reached, -1 if checkDensity/checkPressure
+ * failed for that instance (the other instances continue). */
+typedef struct pdes_batch pdes_batch;
+int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_desc* sbps,
+                      const pdes_phys_desc* phys, pdes_batch** out);
+int pdes_batch_destroy(pdes_batch* batch);
+int pdes_batch_num_groups(pdes_batch* batch, int* ngroups);
+int pdes_batch_set_state(pdes_batch* batch, int inst, const double* q_host);
+int pdes_batch_get_state(pdes_batch* batch, int inst, double* q_host);
+int pdes_batch_newton_solve(pdes_batch* batch, const pdes_newton_opts* opts,
+                            pdes_newton_report* reports);
+/* residual of instance inst at its current state (res_host: its numDof doubles) */
+int pdes_batch_eval_residual(pdes_batch* batch, int inst, double* res_host);
 
 #ifdef __cplusplus
 }

@@ -111,10 +111,18 @@ def jacobian_cs(b, q, rowptr, colind):
     return val
 
 
-def newton_direct(b, q0, Minv, res_abstol=1e-10, itermax=30):
+def newton_direct(b, q0, Minv, res_abstol=1e-10, itermax=30, euler_tau=None, jac=None, scale_euler=False):
     """newtonInner (src/NonlinearSolvers/newton.jl:137-304) with the sparse
     direct solve of jac_type 2 (UMFPACK in the reference, SuperLU here):
-    J dq = -R, q += dq until ||R||_{M^-1} < res_abstol.  Returns (q, norms)."""
+    J dq = -R, q += dq until ||R||_{M^-1} < res_abstol.  Returns (q, norms).
+
+    euler_tau (with jac, the mapping Jacobian per node) turns on the
+    implicit-Euler globalization: calcTauVec
+    (src/NonlinearSolvers/globalization.jl:181-201) gives
+    tau_vec = tau/(1 + sqrt(jac)) per node, applyEuler (:284-320) adds
+    -M/tau_vec to the Jacobian diagonal, updateEuler (:237-262) multiplies
+    tau by ||R_prev||/||R|| at every Jacobian after the first (after the
+    first too, with ||R_prev|| = 1, when scale_euler)."""
     import scipy.sparse as sp
     import scipy.sparse.linalg as spla
     rowptr, colind = sparsity_bsr(b)
@@ -122,6 +130,8 @@ def newton_direct(b, q0, Minv, res_abstol=1e-10, itermax=30):
     nrow = b.mesh.numEl * b.sbp.Np
     q = np.array(q0, dtype=np.float64, copy=True)
     norms = []
+    tau = euler_tau
+    prev = 1.0 if scale_euler else 0.0
     for it in range(itermax + 1):
         r = residual(b, q)
         nrm = float(np.sqrt((r * r * Minv).sum()))
@@ -130,6 +140,13 @@ def newton_direct(b, q0, Minv, res_abstol=1e-10, itermax=30):
             break
         val = jacobian_cs(b, q, rowptr, colind)
         J = sp.bsr_matrix((val, colind, rowptr), shape=(nrow * nd, nrow * nd)).tocsc()
+        if euler_tau is not None:
+            if prev != 0.0:
+                tau = tau * prev / nrm
+            prev = nrm
+            tau_vec = np.repeat((tau / (1.0 + np.sqrt(np.asarray(jac, dtype=np.float64)))).ravel(), nd)
+            M = 1.0 / np.asarray(Minv, dtype=np.float64).ravel()
+            J = (J - sp.diags(M / tau_vec)).tocsc()
         dq = spla.spsolve(J, -r.ravel())
         q = q + dq.reshape(q.shape)
     return q, norms

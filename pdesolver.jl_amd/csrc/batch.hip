@@ -1,0 +1,788 @@
+// batch.hip -- many independent steady solves packed into few launches.
+//
+// Instances that share an SBP operator and physics kind are concatenated into
+// one packed context ("group"): one residual / Jacobian / preconditioner
+// launch covers every instance of the group, driven by an element work list
+// that holds only the instances that have not converged.  Every reduction is
+// segmented per instance over chunks aligned to the instance's first dof, and
+// the GMRES Hessenberg, Givens rotations and Newton tests are kept per
+// instance, so an instance's arithmetic does not depend on which other
+// instances share its group: a solve gives the same bits alone and in a mix.
+// Groups run on their own streams from their own host threads, so the work of
+// different operators overlaps on the device.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "jacobian.hpp"
+
+namespace pdes {
+int pc_setup(pdes_ctx* c, int kind);
+int pc_apply(pdes_ctx* c, const double* b, double* x);
+}  // namespace pdes
+
+namespace {
+
+using namespace pdes;
+
+constexpr int CHUNK = 4096;     // dofs per reduction chunk (aligned to the instance start)
+constexpr int SEG_THREADS = 256;
+
+// ---------------------------------------------------------------------------
+// kernels over the active dofs: launch index t -> element el_list[t / dpe]
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int64_t active_dof(const int32_t* el_list, int64_t t, int dpe) {
+  const int64_t s = t / dpe;
+  return (int64_t)el_list[s] * dpe + (t - s * dpe);
+}
+
+// partial[c][i] = sum over chunk c of V_i . w  (w2: optional weights)
+__global__ void __launch_bounds__(SEG_THREADS)
+seg_dot_partial(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ w,
+                const double* __restrict__ wt, const int64_t* __restrict__ cbeg, const int32_t* __restrict__ clen,
+                const int32_t* __restrict__ clist, double* __restrict__ partial, int mstride) {
+  __shared__ double sh[SEG_THREADS];
+  const int c = clist[blockIdx.x];
+  const int64_t b0 = cbeg[c];
+  const int len = clen[c];
+  for (int i = 0; i < m; ++i) {
+    const double* Vi = V + (int64_t)i * ldv;
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < len; t += SEG_THREADS) {
+      const int64_t k = b0 + t;
+      acc += wt ? Vi[k] * Vi[k] * wt[k] : Vi[k] * w[k];
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = SEG_THREADS / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[(int64_t)c * mstride + i] = sh[0];
+    __syncthreads();
+  }
+}
+
+// out[inst][i] = sum of the instance's chunk partials in chunk order
+__global__ void seg_dot_reduce(const double* __restrict__ partial, int m, int mstride,
+                               const int32_t* __restrict__ inst_c0, const int32_t* __restrict__ ilist,
+                               double* __restrict__ out) {
+  const int inst = ilist[blockIdx.x];
+  const int i = threadIdx.x;
+  if (i >= m) return;
+  double acc = 0.0;
+  for (int c = inst_c0[inst]; c < inst_c0[inst + 1]; ++c) acc += partial[(int64_t)c * mstride + i];
+  out[(int64_t)inst * mstride + i] = acc;
+}
+
+// w -= sum_i coef[inst][i] V_i   over the active dofs
+__global__ void seg_multiaxpy(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ coef,
+                              int mstride, double* __restrict__ w, const int32_t* __restrict__ el_list,
+                              const int32_t* __restrict__ el_inst, int dpe, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t k = active_dof(el_list, t, dpe);
+    const double* cf = coef + (int64_t)el_inst[k / dpe] * mstride;
+    double acc = w[k];
+    for (int i = 0; i < m; ++i) acc -= cf[i] * V[(int64_t)i * ldv + k];
+    w[k] = acc;
+  }
+}
+
+// y = sum_i coef[inst][i] V_i  over the active dofs
+__global__ void seg_combine(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ coef,
+                            int mstride, double* __restrict__ y, const int32_t* __restrict__ el_list,
+                            const int32_t* __restrict__ el_inst, int dpe, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t k = active_dof(el_list, t, dpe);
+    const double* cf = coef + (int64_t)el_inst[k / dpe] * mstride;
+    double acc = 0.0;
+    // basis vectors past an instance's own Krylov dimension are stale: skip them
+    for (int i = 0; i < m; ++i)
+      if (cf[i] != 0.0) acc += cf[i] * V[(int64_t)i * ldv + k];
+    y[k] = acc;
+  }
+}
+
+// y = a[inst] * x (+ y)   over the active dofs
+__global__ void seg_scale(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ a,
+                          int astride, int accumulate, const int32_t* __restrict__ el_list,
+                          const int32_t* __restrict__ el_inst, int dpe, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t k = active_dof(el_list, t, dpe);
+    const double s = a[(int64_t)el_inst[k / dpe] * astride];
+    y[k] = accumulate ? y[k] + s * x[k] : s * x[k];
+  }
+}
+
+unsigned grid_n(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(16384, ceil_div(n, 256))); }
+
+struct InstState {
+  int64_t el0 = 0, el1 = 0;  // element range in the packed mesh
+  bool active = true;        // not yet converged (Newton)
+  bool failed = false;
+  pdes_newton_report rep{};
+  // GMRES state
+  double tau = 0, tau_prev_norm = 0;  // implicit-Euler pseudo-time step and the norm it was last updated at
+  bool g_iter = false;       // still iterating in the current GMRES cycle
+  bool g_done = false;       // converged in GMRES
+  int g_total = 0, g_j = 0;
+  double g_beta0 = -1, g_res = 0;
+  std::vector<double> H, cs, sn, g;
+};
+
+}  // namespace
+
+struct pdes_batch_group {
+  pdes_ctx* c = nullptr;
+  std::vector<int> inst_global;      // global instance index of each local instance
+  std::vector<InstState> inst;
+  std::vector<int32_t> h_el_inst;
+  int dpe = 0;                       // dofs per element
+  // reduction chunks
+  std::vector<int64_t> h_cbeg;
+  std::vector<int32_t> h_clen, h_inst_c0;
+  int64_t* d_cbeg = nullptr;
+  int32_t* d_clen = nullptr;
+  int32_t* d_inst_c0 = nullptr;
+  int32_t* d_clist = nullptr;        // active chunk list
+  int32_t* d_ilist = nullptr;        // active instance list
+  double* d_partial = nullptr;       // [nchunk][mstride]
+  double* d_seg = nullptr;           // [ninst][mstride]
+  double* d_coef = nullptr;          // [ninst][mstride]
+  double* h_seg = nullptr;           // pinned
+  double* h_coef = nullptr;          // pinned
+  int mstride = 0;
+  // solver vectors
+  double *d_V = nullptr, *d_z = nullptr, *d_w = nullptr, *d_dq = nullptr, *d_rhs = nullptr;
+  int restart = 0;
+};
+
+struct pdes_batch {
+  int ninst = 0;
+  std::vector<pdes_batch_group*> groups;
+  std::vector<std::pair<int, int>> where;  // global instance -> (group, local)
+};
+
+namespace {
+
+void group_free(pdes_batch_group* g) {
+  if (!g) return;
+  void* dev[] = {g->d_cbeg, g->d_clen, g->d_inst_c0, g->d_clist, g->d_ilist, g->d_partial, g->d_seg,
+                 g->d_coef, g->d_V, g->d_z, g->d_w, g->d_dq, g->d_rhs};
+  for (void* p : dev)
+    if (p) hipFree(p);
+  if (g->h_seg) hipHostFree(g->h_seg);
+  if (g->h_coef) hipHostFree(g->h_coef);
+  if (g->c) pdes_ctx_destroy(g->c);
+  delete g;
+}
+
+bool same_operator(const pdes_sbp_desc& a, const pdes_sbp_desc& b, int dim) {
+  if (a.Np != b.Np || a.Nf != b.Nf || a.stencil != b.stencil || a.norient != b.norient || a.diag_e != b.diag_e)
+    return false;
+  const int nf = dim + 1;
+  return !std::memcmp(a.w, b.w, a.Np * 8) && !std::memcmp(a.Q, b.Q, (size_t)dim * a.Np * a.Np * 8) &&
+         !std::memcmp(a.interp, b.interp, (size_t)a.stencil * a.Nf * 8) &&
+         !std::memcmp(a.perm, b.perm, (size_t)a.stencil * nf * 4) &&
+         !std::memcmp(a.nbrperm, b.nbrperm, (size_t)a.Nf * a.norient * 4) && !std::memcmp(a.wface, b.wface, a.Nf * 8);
+}
+
+// set the element work list of a group to the instances selected by pick
+template <class Pick>
+int set_worklist(pdes_batch_group* g, Pick pick, std::vector<int>& chunks, std::vector<int>& insts) {
+  std::vector<int32_t> el;
+  chunks.clear();
+  insts.clear();
+  for (size_t i = 0; i < g->inst.size(); ++i) {
+    if (!pick(g->inst[i])) continue;
+    insts.push_back((int)i);
+    for (int64_t e = g->inst[i].el0; e < g->inst[i].el1; ++e) el.push_back((int32_t)e);
+    for (int c = g->h_inst_c0[i]; c < g->h_inst_c0[i + 1]; ++c) chunks.push_back(c);
+  }
+  pdes_ctx* c = g->c;
+  c->nlist = (int32_t)el.size();
+  if (!el.empty()) PDES_HIP(hipMemcpyAsync(c->d_el_list, el.data(), el.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if (!chunks.empty())
+    PDES_HIP(hipMemcpyAsync(g->d_clist, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if (!insts.empty())
+    PDES_HIP(hipMemcpyAsync(g->d_ilist, insts.data(), insts.size() * 4, hipMemcpyHostToDevice, c->stream));
+  // host vectors are reused by the caller until the next call; wait for the copies
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+// seg[inst][0..m) = V_i . w  (or weighted squares of V_0 when wt) for the listed instances
+int seg_dots(pdes_batch_group* g, const double* V, int m, int64_t ldv, const double* w, const double* wt,
+             int nchunks, int ninst_list) {
+  pdes_ctx* c = g->c;
+  if (nchunks == 0) return PDES_OK;
+  hipLaunchKernelGGL(seg_dot_partial, dim3(nchunks), dim3(SEG_THREADS), 0, c->stream, V, m, ldv, w, wt, g->d_cbeg,
+                     g->d_clen, g->d_clist, g->d_partial, g->mstride);
+  hipLaunchKernelGGL(seg_dot_reduce, dim3(ninst_list), dim3(64), 0, c->stream, g->d_partial, m, g->mstride,
+                     g->d_inst_c0, g->d_ilist, g->d_seg);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+int fetch_seg(pdes_batch_group* g) {
+  pdes_ctx* c = g->c;
+  PDES_HIP(hipMemcpyAsync(g->h_seg, g->d_seg, g->inst.size() * g->mstride * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+// The host rewrites h_coef right after each upload, so the upload must have
+// read the pinned buffer before returning: copy, then wait on the stream.
+int push_coef(pdes_batch_group* g) {
+  pdes_ctx* c = g->c;
+  PDES_HIP(hipMemcpyAsync(g->d_coef, g->h_coef, g->inst.size() * g->mstride * sizeof(double), hipMemcpyHostToDevice,
+                          c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int64_t active_dofs(pdes_batch_group* g) { return (int64_t)g->c->nlist * g->dpe; }
+
+int apply_op(pdes_batch_group* g, int mode, const double* x, double* y) {
+  pdes_ctx* c = g->c;
+  if (mode == 0) return launch_spmv(c, x, y, 1.0, nullptr, 0.0);
+  const int st = launch_residual(c, c->d_q, x, nullptr, y, false);
+  return st ? st : launch_ptc_shift(c, x, y);
+}
+
+// Batched GMRES(restart), right preconditioned, x0 = 0, for the instances
+// with active == true; b and x are packed vectors.
+int batch_gmres(pdes_batch_group* g, const pdes_newton_opts* o, const double* b, double* x) {
+  pdes_ctx* c = g->c;
+  const int R = o->gmres_restart;
+  const int64_t ldv = c->numDof;
+  const int ms = g->mstride;
+  std::vector<int> chunks, insts;
+  int st;
+  for (auto& s : g->inst) {
+    s.g_iter = s.active;
+    s.g_done = !s.active;
+    s.g_total = 0;
+    s.g_beta0 = -1;
+    s.H.assign((size_t)(R + 1) * R, 0.0);
+    s.cs.assign(R, 0.0);
+    s.sn.assign(R, 0.0);
+    s.g.assign(R + 1, 0.0);
+  }
+  // x = 0 on the active instances
+  if ((st = set_worklist(g, [](const InstState& s) { return s.g_iter; }, chunks, insts))) return st;
+  bool first = true;
+  while (true) {
+    // r = b - A x  (x = 0 on the first cycle) into V_0, for the instances still iterating
+    if ((st = set_worklist(g, [](const InstState& s) { return !s.g_done; }, chunks, insts))) return st;
+    if (insts.empty()) break;
+    const int64_t n = active_dofs(g);
+    for (int i : insts) g->h_coef[(int64_t)i * ms] = 1.0;
+    if ((st = push_coef(g))) return st;
+    if (first) {
+      for (int i : insts) g->h_coef[(int64_t)i * ms] = 0.0;
+      if ((st = push_coef(g))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, b, x, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n);  // x = 0
+      for (int i : insts) g->h_coef[(int64_t)i * ms] = 1.0;
+      if ((st = push_coef(g))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, b, g->d_V, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n);
+    } else {
+      if ((st = apply_op(g, o->jv_mode, x, g->d_w))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, b, g->d_V, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n);
+      for (int i : insts) g->h_coef[(int64_t)i * ms] = -1.0;
+      if ((st = push_coef(g))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, g->d_w, g->d_V, g->d_coef, ms, 1,
+                         c->d_el_list, c->d_el_inst, g->dpe, n);
+    }
+    first = false;
+    if ((st = seg_dots(g, g->d_V, 1, ldv, g->d_V, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
+    if ((st = fetch_seg(g))) return st;
+    for (int i : insts) {
+      InstState& s = g->inst[i];
+      const double beta = std::sqrt(g->h_seg[(int64_t)i * ms]);
+      if (s.g_beta0 < 0) s.g_beta0 = beta;
+      s.g_res = beta;
+      const double tol = std::max(o->krylov_reltol * s.g_beta0, o->krylov_abstol);
+      s.g_iter = !(beta <= tol || s.g_total >= o->krylov_itermax);
+      if (!s.g_iter) s.g_done = true;
+      std::fill(s.g.begin(), s.g.end(), 0.0);
+      s.g[0] = beta;
+      s.g_j = 0;
+      g->h_coef[(int64_t)i * ms] = s.g_iter ? 1.0 / beta : 0.0;
+    }
+    if ((st = set_worklist(g, [](const InstState& s) { return s.g_iter; }, chunks, insts))) return st;
+    if (insts.empty()) break;
+    if ((st = push_coef(g))) return st;
+    int64_t n2 = active_dofs(g);
+    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n2)), dim3(256), 0, c->stream, g->d_V, g->d_V, g->d_coef, ms, 0,
+                       c->d_el_list, c->d_el_inst, g->dpe, n2);
+    std::vector<int> cycle_insts = insts;  // instances iterating in this cycle
+    for (int j = 0; j < R; ++j) {
+      if ((st = set_worklist(g, [](const InstState& s) { return s.g_iter; }, chunks, insts))) return st;
+      if (insts.empty()) break;
+      n2 = active_dofs(g);
+      double* Vj = g->d_V + (int64_t)j * ldv;
+      double* Vn = g->d_V + (int64_t)(j + 1) * ldv;
+      if ((st = pc_apply(c, Vj, g->d_z))) return st;
+      if ((st = apply_op(g, o->jv_mode, g->d_z, Vn))) return st;
+      // CGS2 with per-instance coefficients
+      std::vector<double> hsum((size_t)g->inst.size() * (j + 1), 0.0);
+      for (int pass = 0; pass < 2; ++pass) {
+        if ((st = seg_dots(g, g->d_V, j + 1, ldv, Vn, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
+        if ((st = fetch_seg(g))) return st;
+        for (int i : insts)
+          for (int k = 0; k <= j; ++k) {
+            const double h = g->h_seg[(int64_t)i * ms + k];
+            g->h_coef[(int64_t)i * ms + k] = h;
+            hsum[(size_t)i * (j + 1) + k] += h;
+          }
+        if ((st = push_coef(g))) return st;
+        hipLaunchKernelGGL(seg_multiaxpy, dim3(grid_n(n2)), dim3(256), 0, c->stream, g->d_V, j + 1, ldv, g->d_coef, ms,
+                           Vn, c->d_el_list, c->d_el_inst, g->dpe, n2);
+      }
+      if ((st = seg_dots(g, Vn, 1, ldv, Vn, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
+      if ((st = fetch_seg(g))) return st;
+      for (int i : insts) {
+        InstState& s = g->inst[i];
+        const double hn = std::sqrt(g->h_seg[(int64_t)i * ms]);
+        for (int k = 0; k <= j; ++k) s.H[(size_t)k * R + j] = hsum[(size_t)i * (j + 1) + k];
+        s.H[(size_t)(j + 1) * R + j] = hn;
+        g->h_coef[(int64_t)i * ms] = hn > 0 ? 1.0 / hn : 0.0;
+        for (int k = 0; k < j; ++k) {
+          const double a = s.H[(size_t)k * R + j], bb = s.H[(size_t)(k + 1) * R + j];
+          s.H[(size_t)k * R + j] = s.cs[k] * a + s.sn[k] * bb;
+          s.H[(size_t)(k + 1) * R + j] = -s.sn[k] * a + s.cs[k] * bb;
+        }
+        const double a = s.H[(size_t)j * R + j], bb = s.H[(size_t)(j + 1) * R + j];
+        const double r = std::hypot(a, bb);
+        s.cs[j] = r > 0 ? a / r : 1.0;
+        s.sn[j] = r > 0 ? bb / r : 0.0;
+        s.H[(size_t)j * R + j] = r;
+        s.H[(size_t)(j + 1) * R + j] = 0.0;
+        s.g[j + 1] = -s.sn[j] * s.g[j];
+        s.g[j] = s.cs[j] * s.g[j];
+        s.g_res = std::fabs(s.g[j + 1]);
+        ++s.g_total;
+        s.g_j = j + 1;
+        const double tol = std::max(o->krylov_reltol * s.g_beta0, o->krylov_abstol);
+        if (s.g_res <= tol || hn == 0.0 || s.g_total >= o->krylov_itermax) s.g_iter = false;
+        if (s.g_res <= tol || hn == 0.0) s.g_done = true;
+      }
+      if ((st = push_coef(g))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n2)), dim3(256), 0, c->stream, Vn, Vn, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n2);
+    }
+    // x += M^-1 V y for the instances of this cycle
+    for (int i : cycle_insts) {
+      InstState& s = g->inst[i];
+      std::vector<double> y(s.g_j);
+      for (int k = s.g_j - 1; k >= 0; --k) {
+        double v = s.g[k];
+        for (int l = k + 1; l < s.g_j; ++l) v -= s.H[(size_t)k * R + l] * y[l];
+        y[k] = v / s.H[(size_t)k * R + k];
+      }
+      for (int k = 0; k < ms; ++k) g->h_coef[(int64_t)i * ms + k] = k < s.g_j ? y[k] : 0.0;
+      if (!s.g_done && s.g_total >= o->krylov_itermax) s.g_done = true;  // iteration limit
+    }
+    {
+      std::vector<int32_t> el;
+      for (int i : cycle_insts)
+        for (int64_t e = g->inst[i].el0; e < g->inst[i].el1; ++e) el.push_back((int32_t)e);
+      c->nlist = (int32_t)el.size();
+      PDES_HIP(hipMemcpyAsync(c->d_el_list, el.data(), el.size() * 4, hipMemcpyHostToDevice, c->stream));
+      if ((st = push_coef(g))) return st;
+      const int64_t n3 = active_dofs(g);
+      hipLaunchKernelGGL(seg_combine, dim3(grid_n(n3)), dim3(256), 0, c->stream, g->d_V, R, ldv, g->d_coef, ms, g->d_w,
+                         c->d_el_list, c->d_el_inst, g->dpe, n3);
+      if ((st = pc_apply(c, g->d_w, g->d_z))) return st;
+      for (int i : cycle_insts) g->h_coef[(int64_t)i * ms] = 1.0;
+      if ((st = push_coef(g))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n3)), dim3(256), 0, c->stream, g->d_z, x, g->d_coef, ms, 1,
+                         c->d_el_list, c->d_el_inst, g->dpe, n3);
+      PDES_HIP(hipStreamSynchronize(c->stream));
+    }
+    bool any = false;
+    for (int i : cycle_insts) any = any || !g->inst[i].g_done;
+    if (!any) break;
+  }
+  for (auto& s : g->inst)
+    if (s.active) s.rep.total_krylov_iters += s.g_total;
+  return PDES_OK;
+}
+
+// ||R_i||_{M^-1} for the active instances (work list must select them); writes rep.res_norm
+int batch_residual_norms(pdes_batch_group* g, std::vector<int>& chunks, std::vector<int>& insts, bool first) {
+  pdes_ctx* c = g->c;
+  int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+  if (st) return st;
+  // weighted squares: seg_dot_partial with wt uses V_0 = res
+  if ((st = seg_dots(g, c->d_res, 1, 0, nullptr, c->d_Minv, (int)chunks.size(), (int)insts.size()))) return st;
+  if ((st = fetch_seg(g))) return st;
+  std::vector<int32_t> flags(g->inst.size(), 0);
+  PDES_HIP(hipMemcpyAsync(flags.data(), c->d_flag, flags.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  for (int i : insts) {
+    InstState& s = g->inst[i];
+    if (flags[i]) {
+      s.failed = true;
+      s.active = false;
+      continue;
+    }
+    const double nrm = std::sqrt(g->h_seg[(int64_t)i * g->mstride]);
+    if (first) s.rep.res_norm0 = nrm;
+    s.rep.res_norm = nrm;
+  }
+  return PDES_OK;
+}
+
+int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
+  pdes_ctx* c = g->c;
+  auto t0 = std::chrono::steady_clock::now();
+  std::vector<int> chunks, insts;
+  int st;
+  PDES_HIP(hipMemsetAsync(c->d_flag, 0, g->inst.size() * 4, c->stream));
+  for (auto& s : g->inst) { s.active = true; s.failed = false; s.rep = pdes_newton_report{}; s.tau = 0; }
+  if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
+  if ((st = batch_residual_norms(g, chunks, insts, true))) return st;
+  for (auto& s : g->inst)
+    if (s.active && s.rep.res_norm < o->res_abstol) { s.active = false; s.rep.converged = 1; }
+  if ((st = jacobian_setup(c, o->fp64_blocks))) return st;
+  for (int it = 1; it <= o->itermax; ++it) {
+    if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
+    if (insts.empty()) break;
+    if (o->globalize_euler) {
+      // per-instance updateEuler (globalization.jl:237-262)
+      std::vector<double> taus(g->inst.size(), 1.0);
+      for (int i : insts) {
+        InstState& s = g->inst[i];
+        if (it == 1)
+          s.tau = o->euler_tau * (o->scale_euler ? 1.0 / s.rep.res_norm : 1.0);
+        else
+          s.tau *= s.tau_prev_norm / s.rep.res_norm;
+        s.tau_prev_norm = s.rep.res_norm;
+      }
+      for (size_t i = 0; i < g->inst.size(); ++i)
+        if (g->inst[i].tau > 0) taus[i] = g->inst[i].tau;
+      if ((st = set_ptc_scale(c, taus.data(), (int)taus.size()))) return st;
+    }
+    if ((st = assemble(c))) return st;
+    if ((st = pc_setup(c, o->pc_kind))) return st;
+    const int64_t n = active_dofs(g);
+    for (int i : insts) g->h_coef[(int64_t)i * g->mstride] = -1.0;
+    if ((st = push_coef(g))) return st;
+    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, c->d_res, g->d_rhs, g->d_coef, g->mstride,
+                       0, c->d_el_list, c->d_el_inst, g->dpe, n);
+    if ((st = batch_gmres(g, o, g->d_rhs, g->d_dq))) return st;
+    // q += dq, step norms, new residual norms for the active instances
+    if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
+    for (int i : insts) g->h_coef[(int64_t)i * g->mstride] = 1.0;
+    if ((st = push_coef(g))) return st;
+    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, g->d_dq, c->d_q, g->d_coef, g->mstride, 1,
+                       c->d_el_list, c->d_el_inst, g->dpe, n);
+    if ((st = seg_dots(g, g->d_dq, 1, 0, g->d_dq, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
+    if ((st = fetch_seg(g))) return st;
+    for (int i : insts) g->inst[i].rep.step_norm = std::sqrt(g->h_seg[(int64_t)i * g->mstride]);
+    if ((st = batch_residual_norms(g, chunks, insts, false))) return st;
+    for (int i : insts) {
+      InstState& s = g->inst[i];
+      if (s.failed) continue;
+      s.rep.iterations = it;
+      if (s.rep.res_norm < o->res_abstol || s.rep.res_norm / s.rep.res_norm0 < o->res_reltol ||
+          s.rep.step_norm <= o->step_tol) {
+        s.rep.converged = 1;
+        s.active = false;
+      }
+    }
+  }
+  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (auto& s : g->inst) s.rep.elapsed_s = el;
+  c->nlist = -1;
+  c->ptc_on = false;
+  return PDES_OK;
+}
+
+}  // namespace
+
+using namespace pdes;
+
+extern "C" {
+
+int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_desc* sbps,
+                      const pdes_phys_desc* phys, pdes_batch** out) {
+  PDES_CHECK(ninst > 0 && meshes && sbps && phys && out, PDES_ERR_ARG, "bad argument");
+  pdes_batch* B = new pdes_batch();
+  B->ninst = ninst;
+  B->where.assign(ninst, {-1, -1});
+  // group instances by operator and physics kind
+  std::vector<std::vector<int>> groups;
+  for (int i = 0; i < ninst; ++i) {
+    bool placed = false;
+    for (auto& gr : groups) {
+      const int r = gr[0];
+      if (meshes[r].dim == meshes[i].dim && phys[r].volume_integral_type == phys[i].volume_integral_type &&
+          phys[r].face_flux == phys[i].face_flux && phys[r].gamma == phys[i].gamma &&
+          phys[r].R_gas == phys[i].R_gas && phys[r].check_positivity == phys[i].check_positivity &&
+          same_operator(sbps[r], sbps[i], meshes[i].dim)) {
+        gr.push_back(i);
+        placed = true;
+        break;
+      }
+    }
+    if (!placed) groups.push_back({i});
+  }
+  int st = PDES_OK;
+  for (size_t gi = 0; gi < groups.size() && !st; ++gi) {
+    const auto& ids = groups[gi];
+    const pdes_mesh_desc& m0 = meshes[ids[0]];
+    const pdes_sbp_desc& s0 = sbps[ids[0]];
+    const int dim = m0.dim, Np = s0.Np, Nf = s0.Nf;
+    // pack
+    std::vector<double> dxidx, jac, nrm_face, nrm_bndry, coords_bndry;
+    std::vector<int32_t> ifaces, bfaces, boff{0}, bct;
+    std::vector<int64_t> el_start{0};
+    int64_t nel = 0, nif = 0, nbf = 0;
+    for (int id : ids) {
+      const pdes_mesh_desc& m = meshes[id];
+      dxidx.insert(dxidx.end(), m.dxidx, m.dxidx + (size_t)m.numEl * Np * dim * dim);
+      jac.insert(jac.end(), m.jac, m.jac + (size_t)m.numEl * Np);
+      for (int k = 0; k < m.numInterfaces; ++k) {
+        const int32_t* it = m.interfaces + 5 * k;
+        ifaces.insert(ifaces.end(), {(int32_t)(it[0] + nel), (int32_t)(it[1] + nel), it[2], it[3], it[4]});
+      }
+      nrm_face.insert(nrm_face.end(), m.nrm_face, m.nrm_face + (size_t)m.numInterfaces * Nf * dim);
+      for (int k = 0; k < m.numBoundaryFaces; ++k)
+        bfaces.insert(bfaces.end(), {(int32_t)(m.bndryfaces[2 * k] + nel), m.bndryfaces[2 * k + 1]});
+      nrm_bndry.insert(nrm_bndry.end(), m.nrm_bndry, m.nrm_bndry + (size_t)m.numBoundaryFaces * Nf * dim);
+      coords_bndry.insert(coords_bndry.end(), m.coords_bndry, m.coords_bndry + (size_t)m.numBoundaryFaces * Nf * dim);
+      for (int bc = 0; bc < m.numBC; ++bc) {
+        boff.push_back((int32_t)(m.bndry_offsets[bc + 1] + nbf));
+        bct.push_back(m.bc_type[bc]);
+      }
+      nel += m.numEl;
+      nif += m.numInterfaces;
+      nbf += m.numBoundaryFaces;
+      el_start.push_back(nel);
+    }
+    PDES_CHECK(nel < (int64_t)1 << 31, PDES_ERR_ARG, "packed group too large");
+    pdes_mesh_desc pm{dim, (int32_t)nel, (int32_t)nif, (int32_t)nbf, (int32_t)bct.size(), dxidx.data(), jac.data(),
+                      ifaces.data(), bfaces.data(), boff.data(), bct.data(), nrm_face.data(), nrm_bndry.data(),
+                      coords_bndry.data()};
+    pdes_batch_group* g = new pdes_batch_group();
+    B->groups.push_back(g);
+    st = pdes_ctx_create(&pm, &s0, &phys[ids[0]], &g->c);
+    if (st) break;
+    pdes_ctx* c = g->c;
+    const int ni = (int)ids.size();
+    c->ninst = ni;
+    g->dpe = Np * c->nd;
+    g->inst_global = ids;
+    g->inst.resize(ni);
+    g->h_el_inst.resize(nel);
+    for (int i = 0; i < ni; ++i) {
+      g->inst[i].el0 = el_start[i];
+      g->inst[i].el1 = el_start[i + 1];
+      for (int64_t e = el_start[i]; e < el_start[i + 1]; ++e) g->h_el_inst[e] = i;
+      B->where[ids[i]] = {(int)gi, i};
+    }
+    // per-instance physics and boundary records tagged with the instance
+    std::vector<PhysConst> ip(ni);
+    for (int i = 0; i < ni; ++i) {
+      const pdes_phys_desc& p = phys[ids[i]];
+      ip[i] = PhysConst{p.gamma, p.gamma - 1, p.R_gas, p.R_gas / (p.gamma - 1), p.Ma, p.aoa, p.rho_free,
+                        p.p_free / (p.gamma - 1) + 0.5 * p.rho_free * p.Ma * p.Ma, p.vortex_Min};
+    }
+    for (int64_t e = 0; e < nel; ++e)
+      for (int f = 0; f < c->nfaces; ++f) {
+        ElFace& r = c->h_elface[(size_t)e * c->nfaces + f];
+        if (r.x < 0) r.w = (r.w & 0xff) | (g->h_el_inst[e] << 8);
+      }
+    // reduction chunks aligned to each instance's first dof
+    g->h_inst_c0.push_back(0);
+    for (int i = 0; i < ni; ++i) {
+      const int64_t d0 = el_start[i] * g->dpe, d1 = el_start[i + 1] * g->dpe;
+      for (int64_t d = d0; d < d1; d += CHUNK) {
+        g->h_cbeg.push_back(d);
+        g->h_clen.push_back((int32_t)std::min<int64_t>(CHUNK, d1 - d));
+      }
+      g->h_inst_c0.push_back((int32_t)g->h_cbeg.size());
+    }
+    g->restart = 30;
+    g->mstride = 64;
+    const size_t nchunk = g->h_cbeg.size();
+    auto dm = [&](void** p, size_t bytes) {
+      if (st) return;
+      if (hipMalloc(p, std::max<size_t>(bytes, 8)) != hipSuccess) {
+        set_error("batch allocation failed");
+        st = PDES_ERR_NOMEM;
+      }
+    };
+    hipFree(c->d_flag);
+    c->d_flag = nullptr;
+    dm((void**)&c->d_flag, ni * 4);
+    dm((void**)&c->d_el_list, nel * 4);
+    dm((void**)&c->d_el_inst, nel * 4);
+    dm((void**)&c->d_inst_phys, ni * sizeof(PhysConst));
+    hipFree(c->d_ptc_scale);
+    c->d_ptc_scale = nullptr;
+    dm((void**)&c->d_ptc_scale, ni * 8);
+    dm((void**)&g->d_cbeg, nchunk * 8);
+    dm((void**)&g->d_clen, nchunk * 4);
+    dm((void**)&g->d_inst_c0, (ni + 1) * 4);
+    dm((void**)&g->d_clist, nchunk * 4);
+    dm((void**)&g->d_ilist, ni * 4);
+    dm((void**)&g->d_partial, nchunk * g->mstride * 8);
+    dm((void**)&g->d_seg, (size_t)ni * g->mstride * 8);
+    dm((void**)&g->d_coef, (size_t)ni * g->mstride * 8);
+    dm((void**)&g->d_z, c->numDof * 8);
+    dm((void**)&g->d_w, c->numDof * 8);
+    dm((void**)&g->d_dq, c->numDof * 8);
+    dm((void**)&g->d_rhs, c->numDof * 8);
+    if (st) break;
+    if (hipHostMalloc((void**)&g->h_seg, (size_t)ni * g->mstride * 8) != hipSuccess ||
+        hipHostMalloc((void**)&g->h_coef, (size_t)ni * g->mstride * 8) != hipSuccess) {
+      set_error("pinned allocation failed");
+      st = PDES_ERR_NOMEM;
+      break;
+    }
+    std::memset(g->h_coef, 0, (size_t)ni * g->mstride * 8);
+    hipStream_t s = c->stream;
+    if (hipMemcpyAsync(c->d_elface, c->h_elface.data(), c->h_elface.size() * sizeof(ElFace), hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(c->d_el_inst, g->h_el_inst.data(), nel * 4, hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(c->d_inst_phys, ip.data(), ni * sizeof(PhysConst), hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(g->d_cbeg, g->h_cbeg.data(), nchunk * 8, hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(g->d_clen, g->h_clen.data(), nchunk * 4, hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(g->d_inst_c0, g->h_inst_c0.data(), (ni + 1) * 4, hipMemcpyHostToDevice, s) ||
+        hipMemsetAsync(c->d_flag, 0, ni * 4, s) || hipStreamSynchronize(s)) {
+      set_error("batch upload failed");
+      st = PDES_ERR_HIP;
+    }
+  }
+  if (st) {
+    for (auto* g : B->groups) group_free(g);
+    delete B;
+    return st;
+  }
+  *out = B;
+  return PDES_OK;
+}
+
+int pdes_batch_destroy(pdes_batch* B) {
+  if (!B) return PDES_OK;
+  for (auto* g : B->groups) group_free(g);
+  delete B;
+  return PDES_OK;
+}
+
+int pdes_batch_num_groups(pdes_batch* B, int* ngroups) {
+  PDES_CHECK(B && ngroups, PDES_ERR_ARG, "null argument");
+  *ngroups = (int)B->groups.size();
+  return PDES_OK;
+}
+
+static int locate(pdes_batch* B, int inst, pdes_batch_group** g, InstState** s) {
+  PDES_CHECK(B && inst >= 0 && inst < B->ninst, PDES_ERR_ARG, "instance %d out of range", inst);
+  auto w = B->where[inst];
+  *g = B->groups[w.first];
+  *s = &(*g)->inst[w.second];
+  return PDES_OK;
+}
+
+int pdes_batch_set_state(pdes_batch* B, int inst, const double* q) {
+  pdes_batch_group* g;
+  InstState* s;
+  int st = locate(B, inst, &g, &s);
+  if (st) return st;
+  PDES_CHECK(q, PDES_ERR_ARG, "null state");
+  const int64_t off = s->el0 * g->dpe, n = (s->el1 - s->el0) * g->dpe;
+  PDES_HIP(hipMemcpyAsync(g->c->d_q + off, q, n * 8, hipMemcpyHostToDevice, g->c->stream));
+  PDES_HIP(hipStreamSynchronize(g->c->stream));
+  return PDES_OK;
+}
+
+int pdes_batch_get_state(pdes_batch* B, int inst, double* q) {
+  pdes_batch_group* g;
+  InstState* s;
+  int st = locate(B, inst, &g, &s);
+  if (st) return st;
+  PDES_CHECK(q, PDES_ERR_ARG, "null state");
+  const int64_t off = s->el0 * g->dpe, n = (s->el1 - s->el0) * g->dpe;
+  PDES_HIP(hipMemcpyAsync(q, g->c->d_q + off, n * 8, hipMemcpyDeviceToHost, g->c->stream));
+  PDES_HIP(hipStreamSynchronize(g->c->stream));
+  return PDES_OK;
+}
+
+// residual of every instance (one launch per group), instance inst copied out;
+// returns PDES_ERR_PHYS if that instance fails checkDensity/checkPressure
+int pdes_batch_eval_residual(pdes_batch* B, int inst, double* res) {
+  pdes_batch_group* g;
+  InstState* s;
+  int st = locate(B, inst, &g, &s);
+  if (st) return st;
+  PDES_CHECK(res, PDES_ERR_ARG, "null output");
+  pdes_ctx* c = g->c;
+  const int local = B->where[inst].second;
+  c->nlist = -1;
+  PDES_HIP(hipMemsetAsync(c->d_flag, 0, g->inst.size() * 4, c->stream));
+  st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+  if (st) return st;
+  const int64_t off = s->el0 * g->dpe, n = (s->el1 - s->el0) * g->dpe;
+  int32_t flag = 0;
+  PDES_HIP(hipMemcpyAsync(res, c->d_res + off, n * 8, hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipMemcpyAsync(&flag, c->d_flag + local, 4, hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  PDES_CHECK(!flag, PDES_ERR_PHYS, "instance %d: non-positive density or pressure", inst);
+  return PDES_OK;
+}
+
+// Newton for every instance of the batch; the groups run concurrently, each
+// from its own host thread on its own stream.  reports[ninst] (may be null).
+int pdes_batch_newton_solve(pdes_batch* B, const pdes_newton_opts* o, pdes_newton_report* reports) {
+  PDES_CHECK(B && o, PDES_ERR_ARG, "null argument");
+  PDES_CHECK(o->gmres_restart > 0 && o->gmres_restart < 63, PDES_ERR_ARG, "gmres_restart must be in 1..62");
+  PDES_CHECK(!o->globalize_euler || o->euler_tau > 0, PDES_ERR_ARG, "euler_tau must be positive");
+  std::vector<int> status(B->groups.size(), 0);
+  std::vector<std::string> errs(B->groups.size());
+  std::vector<std::thread> th;
+  for (size_t gi = 0; gi < B->groups.size(); ++gi) {
+    pdes_batch_group* g = B->groups[gi];
+    if (g->restart != o->gmres_restart || !g->d_V) {
+      if (g->d_V) hipFree(g->d_V);
+      g->d_V = nullptr;
+      g->restart = o->gmres_restart;
+      if (hipMalloc((void**)&g->d_V, (size_t)(g->restart + 1) * g->c->numDof * 8) != hipSuccess ||
+          hipMemsetAsync(g->d_V, 0, (size_t)(g->restart + 1) * g->c->numDof * 8, g->c->stream) != hipSuccess ||
+          hipStreamSynchronize(g->c->stream) != hipSuccess) {
+        set_error("GMRES basis allocation failed");
+        return PDES_ERR_NOMEM;
+      }
+    }
+    th.emplace_back([&, gi, g]() {
+      status[gi] = group_newton(g, o);
+      if (status[gi]) errs[gi] = pdes_last_error();
+    });
+  }
+  for (auto& t : th) t.join();
+  for (size_t gi = 0; gi < B->groups.size(); ++gi)
+    if (status[gi]) {
+      set_error("%s", errs[gi].c_str());
+      return status[gi];
+    }
+  if (reports)
+    for (int i = 0; i < B->ninst; ++i) {
+      auto w = B->where[i];
+      reports[i] = B->groups[w.first]->inst[w.second].rep;
+      if (B->groups[w.first]->inst[w.second].failed) reports[i].converged = -1;
+    }
+  return PDES_OK;
+}
+
+}  // extern "C"

@@ -1,5 +1,6 @@
 // ctx.hip -- context lifetime, table upload and state transfer (C ABI).
 #include <cstdarg>
+#include <cmath>
 #include <cstring>
 
 #include "ctx.hpp"
@@ -156,6 +157,16 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
       for (int k = 0; k < c->nd; ++k) Minv[((size_t)e * Np + j) * c->nd + k] = v;
     }
 
+  // globalization diagonal M_i (1 + sqrt(jac_i)) = (w/jac)(1 + sqrt(jac)): with
+  // tau_vec = tau/(1 + sqrt(jac)) (calcTauVec, globalization.jl:181-201) the
+  // term M/tau_vec is this vector times 1/tau
+  std::vector<double> ptc((size_t)m->numEl * Np);
+  for (int e = 0; e < m->numEl; ++e)
+    for (int j = 0; j < Np; ++j) {
+      const double jac = m->jac[(size_t)e * Np + j];
+      ptc[(size_t)e * Np + j] = (s->w[j] / jac) * (1.0 + std::sqrt(jac));
+    }
+
   int st = PDES_OK;
   hipStream_t sm = c->stream;
   const size_t nI = m->numInterfaces, nB = m->numBoundaryFaces;
@@ -190,6 +201,8 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   if (!st) st = upload(&c->d_nodepos, c->h_nodepos.data(), c->h_nodepos.size(), sm);
   if (!st) st = upload(&c->d_wface, s->wface, (size_t)Nf, sm);
   if (!st) st = upload(&c->d_Minv, Minv.data(), Minv.size(), sm);
+  if (!st) st = upload(&c->d_ptc_diag, ptc.data(), ptc.size(), sm);
+  if (!st) st = upload<double>(&c->d_ptc_scale, nullptr, 1, sm);
   if (!st) st = upload<double>(&c->d_q, nullptr, (size_t)c->numDof, sm);
   if (!st) st = upload<double>(&c->d_res, nullptr, (size_t)c->numDof, sm);
   if (!st) st = upload<double>(&c->d_work, nullptr, (size_t)c->numDof, sm);
@@ -214,7 +227,8 @@ int pdes_ctx_destroy(pdes_ctx* c) {
   precond_free(c);
   void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q,
                   c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_q,
-                  c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag};
+                  c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
+                  c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);

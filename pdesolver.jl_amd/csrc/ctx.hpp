@@ -41,7 +41,18 @@ struct pdes_ctx {
   double* d_work2 = nullptr;    // numDof scratch
   double* d_partial = nullptr;  // reduction partials
   double* d_scalar = nullptr;   // reduction results
-  int32_t* d_flag = nullptr;    // positivity flag
+  int32_t* d_flag = nullptr;    // positivity flag, one per instance
+  // packed batches: element work list (active instances), element -> instance,
+  // per-instance physics; single-instance contexts leave these unset
+  int32_t* d_el_list = nullptr;
+  int32_t nlist = -1;           // < 0: no list, every element is processed
+  int32_t* d_el_inst = nullptr;
+  pdes::PhysConst* d_inst_phys = nullptr;
+  int ninst = 1;
+  // implicit-Euler globalization: per node M (1 + sqrt(jac)), per instance 1/tau
+  double* d_ptc_diag = nullptr;
+  double* d_ptc_scale = nullptr;
+  bool ptc_on = false;
   int partial_n = 0;
   // residual launch variants (see residual_launch.hpp) for the plain and the
   // RK-epilogue kernel, chosen by the autotuner or the default table
@@ -52,8 +63,12 @@ struct pdes_ctx {
   void* gmres_ws = nullptr;     // cached GMRES workspace
 
   pdes::DevMesh devmesh() const {
-    return pdes::DevMesh{numEl, metric_per_node, d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};
+    return pdes::DevMesh{numEl, metric_per_node, nlist >= 0 ? d_el_list : nullptr, nlist, d_el_inst, d_inst_phys,
+                         d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};
   }
+  // elements processed by a launch (the work list when one is set)
+  int64_t active_count() const { return nlist >= 0 ? nlist : numEl; }
+  const int32_t* active_list() const { return nlist >= 0 ? d_el_list : nullptr; }
   pdes::DevSBP devsbp() const {
     return pdes::DevSBP{d_Q, d_interp, d_perm, d_nbrperm, d_nodepos, d_wface, norient};
   }

@@ -64,9 +64,10 @@ static void build_pattern(const pdes_ctx* c, std::vector<int64_t>& rowptr, std::
 template <class Op, int VOL, int FLUX, class BT>
 static int launch_jac(pdes_ctx* c) {
   pdes_jacobian* j = c->jac;
-  hipLaunchKernelGGL((euler_jacobian_kernel<Op, VOL, FLUX, BT>), dim3(c->numEl), dim3(256), 0, c->stream,
+  if (c->active_count() == 0) return PDES_OK;
+  hipLaunchKernelGGL((euler_jacobian_kernel<Op, VOL, FLUX, BT>), dim3((unsigned)c->active_count()), dim3(256), 0, c->stream,
                      c->devmesh(), c->devsbp(), c->pc, c->d_q, j->d_rowptr, j->d_colind,
-                     (BT*)j->d_vals, j->d_err);
+                     (BT*)j->d_vals, j->d_err, c->d_ptc_diag, c->ptc_on ? c->d_ptc_scale : nullptr);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -98,15 +99,55 @@ int launch_assemble(pdes_ctx* c) {
 template <int ND>
 static int launch_spmv_nd(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta) {
   pdes_jacobian* j = c->jac;
-  const int64_t n = j->nrows * ND;
+  const int64_t nrows = c->active_count() * c->Np;
+  if (nrows == 0) return PDES_OK;
+  const int64_t n = nrows * ND;
   const int threads = 256;
   const int64_t blocks = ceil_div(n, threads);
   if (j->fp64)
-    hipLaunchKernelGGL((bsr_spmv_kernel<ND, double>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, j->nrows,
-                       j->d_rowptr, j->d_colind, (const double*)j->d_vals, x, y, alpha, yadd, beta);
+    hipLaunchKernelGGL((bsr_spmv_kernel<ND, double>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, nrows,
+                       j->d_rowptr, j->d_colind, (const double*)j->d_vals, x, y, alpha, yadd, beta,
+                       c->active_list(), c->Np);
   else
-    hipLaunchKernelGGL((bsr_spmv_kernel<ND, float>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, j->nrows,
-                       j->d_rowptr, j->d_colind, (const float*)j->d_vals, x, y, alpha, yadd, beta);
+    hipLaunchKernelGGL((bsr_spmv_kernel<ND, float>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, nrows,
+                       j->d_rowptr, j->d_colind, (const float*)j->d_vals, x, y, alpha, yadd, beta,
+                       c->active_list(), c->Np);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+// y -= s_inst * ptc_diag[node] * x over the active elements (the implicit-Euler
+// term of a matrix-free operator, applyEuler for vectors in globalization.jl)
+__global__ void ptc_shift_kernel(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ ptc_diag,
+                                 const double* __restrict__ ptc_scale, const int32_t* __restrict__ el_inst,
+                                 const int32_t* __restrict__ el_list, int64_t nel, int np, int nd) {
+  const int64_t n = nel * np * nd;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t s = t / (np * nd);
+    const int64_t e = el_list ? (int64_t)el_list[s] : s;
+    const int64_t k = e * np * nd + (t - s * np * nd);
+    const double sc = ptc_scale[el_inst ? el_inst[e] : 0];
+    y[k] -= sc * ptc_diag[k / nd] * x[k];
+  }
+}
+
+int set_ptc_scale(pdes_ctx* c, const double* tau, int n) {
+  std::vector<double> inv(n);
+  for (int i = 0; i < n; ++i) inv[i] = 1.0 / tau[i];
+  PDES_HIP(hipMemcpyAsync(c->d_ptc_scale, inv.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  c->ptc_on = true;
+  return PDES_OK;
+}
+
+int launch_ptc_shift(pdes_ctx* c, const double* x, double* y) {
+  if (!c->ptc_on) return PDES_OK;
+  const int64_t n = c->active_count() * c->Np * c->nd;
+  if (n == 0) return PDES_OK;
+  hipLaunchKernelGGL(ptc_shift_kernel, dim3((unsigned)std::min<int64_t>(16384, ceil_div(n, 256))), dim3(256), 0,
+                     c->stream, x, y, c->d_ptc_diag, c->d_ptc_scale, c->d_el_inst, c->active_list(),
+                     c->active_count(), c->Np, c->nd);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }

@@ -22,4 +22,8 @@ int jacobian_setup(pdes_ctx* c, int fp64);
 int assemble(pdes_ctx* c);
 // y = alpha J x + beta yadd (yadd may be null)
 int launch_spmv(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta);
+// y -= (M / tau_vec) x for the implicit-Euler globalization (no-op when off)
+int launch_ptc_shift(pdes_ctx* c, const double* x, double* y);
+// per-instance pseudo-time steps tau[0..n) (stored as 1/tau); turns the term on
+int set_ptc_scale(pdes_ctx* c, const double* tau, int n);
 }  // namespace pdes

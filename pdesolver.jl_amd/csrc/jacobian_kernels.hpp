@@ -58,13 +58,14 @@ template <class Op, int VOL, int FLUX, class BT>
 __global__ void __launch_bounds__(256)
 euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __restrict__ qin,
                       const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-                      BT* __restrict__ vals, int32_t* __restrict__ err) {
+                      BT* __restrict__ vals, int32_t* __restrict__ err,
+                      const double* __restrict__ ptc_diag, const double* __restrict__ ptc_scale) {
   constexpr int DIM = Op::DIM, NP = Op::NP, NF = Op::NF, ST = Op::ST, ND = Op::ND, NFACES = Op::NFACES;
   constexpr int NB = ND * ND;
   using Smem = JacSmem<Op, VOL, FLUX>;
   __shared__ Smem sm;
   const int tid = threadIdx.x;
-  const int64_t e = blockIdx.x;
+  const int64_t e = element_of(mesh, blockIdx.x);
   for (int x = tid; x < DIM * NP * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.Q[x];
   for (int x = tid; x < ST * NF; x += blockDim.x) (&sm.interp[0][0])[x] = sbp.interp[x];
   for (int x = tid; x < ST * NFACES; x += blockDim.x) (&sm.perm[0][0])[x] = sbp.perm[x];
@@ -127,10 +128,10 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
       }
       double qg[ND];
       if constexpr (DIM == 2) {
-        if (r.w == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(phys, mesh.coords_bndry + ((int64_t)r.y * NF + k) * DIM, qg);
-        else free_stream<DIM>(phys, qg);
+        if (bc_kind(r) == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(bc_phys(mesh, phys, r), mesh.coords_bndry + ((int64_t)r.y * NF + k) * DIM, qg);
+        else free_stream<DIM>(bc_phys(mesh, phys, r), qg);
       } else {
-        free_stream<DIM>(phys, qg);
+        free_stream<DIM>(bc_phys(mesh, phys, r), qg);
       }
 #pragma unroll
       for (int c = 0; c < ND; ++c) qR[c] = dual<2 * ND>(qg[c]);
@@ -145,15 +146,22 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
       const int side = (r.z >> 16) & 1;
       const int kR = sm.kown[f][k] == k ? sm.knbr[f][k] : sm.kown[f][k];
       const int fL = side == 0 ? f : fN, fR = side == 0 ? fN : f;
-      const double* qLsrc = side == 0 ? nullptr : qin + nb * NP * ND;
-      const double* qRsrc = side == 0 ? qin + nb * NP * ND : nullptr;
+      const double* qnb = qin + nb * NP * ND;
 #pragma unroll
       for (int c = 0; c < ND; ++c) {
         double vl = 0.0, vr = 0.0;
-        for (int s = 0; s < ST; ++s) {
-          const int nl = sm.perm[s][fL], nr = sm.perm[s][fR];
-          vl += sm.interp[s][k] * (qLsrc ? qLsrc[nl * ND + c] : sm.q[nl][c]);
-          vr += sm.interp[s][kR] * (qRsrc ? qRsrc[nr * ND + c] : sm.q[nr][c]);
+        // own side from LDS, neighbour side from global memory (separate
+        // branches: no pointer select across address spaces)
+        if (side == 0) {
+          for (int s = 0; s < ST; ++s) {
+            vl += sm.interp[s][k] * sm.q[sm.perm[s][fL]][c];
+            vr += sm.interp[s][kR] * qnb[sm.perm[s][fR] * ND + c];
+          }
+        } else {
+          for (int s = 0; s < ST; ++s) {
+            vl += sm.interp[s][k] * qnb[sm.perm[s][fL] * ND + c];
+            vr += sm.interp[s][kR] * sm.q[sm.perm[s][fR]][c];
+          }
         }
         qL[c] = dual<2 * ND>(vl);
         qL[c].d[c] = 1.0;
@@ -257,6 +265,13 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
           }
         }
       }
+      // implicit-Euler globalization (applyEuler, globalization.jl): -M/tau_vec on the diagonal
+      if (ptc_scale && i == j) {
+        const double s = ptc_scale[mesh.el_inst ? mesh.el_inst[e] : 0];
+        const double d = ptc_diag[e * NP + i] * s;
+#pragma unroll
+        for (int a = 0; a < ND; ++a) B[a][a] -= d;
+      }
     } else {
       const int tt = t - NP * NP;
       const int f = tt / NBFACE, u = tt - f * NBFACE;
@@ -309,11 +324,15 @@ template <int ND, class BT>
 __global__ void __launch_bounds__(256)
 bsr_spmv_kernel(int64_t nrows, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
                 const BT* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y,
-                double alpha, const double* __restrict__ yadd, double beta) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t row = t / ND;
-  const int a = (int)(t - row * ND);
-  if (row >= nrows) return;
+                double alpha, const double* __restrict__ yadd, double beta,
+                const int32_t* __restrict__ el_list, int np) {
+  const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t rr = tt / ND;
+  const int a = (int)(tt - rr * ND);
+  if (rr >= nrows) return;
+  // with an element list, launch row rr is node (rr % np) of element el_list[rr / np]
+  const int64_t row = el_list ? (int64_t)el_list[rr / np] * np + (rr % np) : rr;
+  const int64_t t = row * ND + a;
   double acc = 0.0;
   const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];
   for (int64_t b = b0; b < b1; ++b) {

@@ -27,9 +27,19 @@ namespace pdes {
 //   w: BC functor of a boundary face (PDES_BC_*)
 struct ElFace { int32_t x, y, z, w; };
 
+struct PhysConst;
+
 struct DevMesh {
   int32_t numEl;
   int32_t metric_per_node;                // 1: dxidx[e][j][x][xi]; 0: affine, dxidx[e][x][xi]
+  // work queue: when el_list is set, slot s of the launch processes element
+  // el_list[s], s < nlist (packed batches run only the unconverged instances)
+  const int32_t* __restrict__ el_list;
+  int32_t nlist;
+  // packed batches: instance of each element (positivity flags) and the
+  // per-instance physics used by boundary states (index in ElFace.w >> 8)
+  const int32_t* __restrict__ el_inst;
+  const PhysConst* __restrict__ inst_phys;
   const double* __restrict__ dxidx;
   const ElFace* __restrict__ elface;      // [e][f]
   const double* __restrict__ nrm_face;    // [iface][k][dim]
@@ -46,6 +56,18 @@ struct DevSBP {
   const double* __restrict__ wface;    // [k]
   int32_t norient;
 };
+
+__device__ __forceinline__ int32_t launch_count(const DevMesh& m) { return m.el_list ? m.nlist : m.numEl; }
+__device__ __forceinline__ int64_t element_of(const DevMesh& m, int64_t slot) {
+  return m.el_list ? (int64_t)m.el_list[slot] : slot;
+}
+// boundary functor and its physics (per instance in a packed batch)
+__device__ __forceinline__ int bc_kind(const ElFace& r) { return r.w & 0xff; }
+template <class P>
+__device__ __forceinline__ P bc_phys(const DevMesh& m, const P& phys, const ElFace& r) {
+  if (m.inst_phys) return m.inst_phys[r.w >> 8];
+  return phys;
+}
 
 // the DIM x DIM metric block [x][xi] of node j of element e
 template <int DIM, int NP>
@@ -125,8 +147,9 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 
   const int el = tid / NP;
   const int i = tid - el * NP;
-  const int64_t e = (int64_t)blockIdx.x * EPB + el;
-  const bool active = (el < EPB) && (e < mesh.numEl);
+  const int64_t eslot = (int64_t)blockIdx.x * EPB + el;
+  const bool active = (el < EPB) && (eslot < launch_count(mesh));
+  const int64_t e = active ? element_of(mesh, eslot) : 0;
 
   T qi[ND];
   if (active) {
@@ -137,7 +160,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     }
     // checkDensity / checkPressure (euler.jl:543-611)
     if (flag) {
-      if (!(re(qi[0]) > 0.0) || !(re(pressure<DIM>(phys, qi)) > 0.0)) flag[0] = 1;
+      if (!(re(qi[0]) > 0.0) || !(re(pressure<DIM>(phys, qi)) > 0.0)) flag[mesh.el_inst ? mesh.el_inst[e] : 0] = 1;
     }
   }
   double dxi[DIM][DIM];  // dxidx[e][i][x][xi]
@@ -314,10 +337,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           for (int c = 0; c < ND; ++c) qL[c] = sm.q[el][node][c];
           double qg[ND];
           if constexpr (DIM == 2) {
-            if (rec.w == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(phys, mesh.coords_bndry + ((int64_t)rec.y * NF + k) * DIM, qg);
-            else free_stream<DIM>(phys, qg);
+            if (bc_kind(rec) == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(bc_phys(mesh, phys, rec), mesh.coords_bndry + ((int64_t)rec.y * NF + k) * DIM, qg);
+            else free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
           } else {
-            free_stream<DIM>(phys, qg);
+            free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
           }
 #pragma unroll
           for (int c = 0; c < ND; ++c) qR[c] = qg[c];
@@ -377,10 +400,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         const int64_t b = rec.y;
         double qg[ND];
         if constexpr (DIM == 2) {
-          if (rec.w == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(phys, mesh.coords_bndry + (b * NF + k) * DIM, qg);
-          else free_stream<DIM>(phys, qg);
+          if (bc_kind(rec) == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(bc_phys(mesh, phys, rec), mesh.coords_bndry + (b * NF + k) * DIM, qg);
+          else free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
         } else {
-          free_stream<DIM>(phys, qg);
+          free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
         }
 #pragma unroll
         for (int c = 0; c < ND; ++c) qR[c] = qg[c];

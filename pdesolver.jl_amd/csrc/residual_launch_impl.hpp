@@ -10,7 +10,8 @@ template <class Op, int VOL, int FLUX, class T, int EPB, int WAVES>
 static int launch_cfg(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, int32_t* flag,
                       const RKStage* rk) {
   const int threads = EPB * Op::NP;
-  const int blocks = (int)ceil_div(c->numEl, EPB);
+  if (c->active_count() == 0) return PDES_OK;
+  const int blocks = (int)ceil_div(c->active_count(), EPB);
   if constexpr (std::is_same<T, double>::value) {
     if (rk) {
       hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T, true, WAVES>), dim3(blocks), dim3(threads), 0,

@@ -43,16 +43,16 @@ int precond_free(pdes_ctx* c) {
 // ---------------------------------------------------------------------------
 template <int NP, int ND, class BT>
 __global__ void __launch_bounds__(256)
-bjacobi_factor_kernel(int numEl, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
-                      const BT* __restrict__ vals, float* __restrict__ lu, int32_t* __restrict__ piv,
-                      int32_t* __restrict__ err) {
+bjacobi_factor_kernel(const int32_t* __restrict__ el_list, const int64_t* __restrict__ rowptr,
+                      const int32_t* __restrict__ colind, const BT* __restrict__ vals, float* __restrict__ lu,
+                      int32_t* __restrict__ piv, int32_t* __restrict__ err) {
   constexpr int N = NP * ND;
   __shared__ float A[N][N + 1];
   __shared__ float redv[64];
   __shared__ int redi[64];
   __shared__ int pivrow;
   const int tid = threadIdx.x;
-  const int64_t e = blockIdx.x;
+  const int64_t e = el_list ? (int64_t)el_list[blockIdx.x] : (int64_t)blockIdx.x;
   for (int t = tid; t < NP * NP; t += blockDim.x) {
     const int i = t / NP, j = t - i * NP;
     const int64_t row = e * NP + i;
@@ -113,12 +113,12 @@ bjacobi_factor_kernel(int numEl, const int64_t* __restrict__ rowptr, const int32
 // block-Jacobi apply: one wave per element, lane r owns rows r, r+64, ...
 template <int NP, int ND>
 __global__ void __launch_bounds__(64)
-bjacobi_apply_kernel(int numEl, const float* __restrict__ lu, const int32_t* __restrict__ piv,
-                     const double* __restrict__ b, double* __restrict__ x) {
+bjacobi_apply_kernel(const int32_t* __restrict__ el_list, const float* __restrict__ lu,
+                     const int32_t* __restrict__ piv, const double* __restrict__ b, double* __restrict__ x) {
   constexpr int N = NP * ND;
   __shared__ float v[N];
   const int lane = threadIdx.x;
-  const int64_t e = blockIdx.x;
+  const int64_t e = el_list ? (int64_t)el_list[blockIdx.x] : (int64_t)blockIdx.x;
   for (int r = lane; r < N; r += 64) v[r] = (float)b[e * N + r];
   __syncthreads();
   if (lane == 0)
@@ -147,11 +147,13 @@ template <int NP, int ND>
 static int bj_factor(pdes_ctx* c) {
   pdes_jacobian* j = c->jac;
   pdes_precond* p = c->pc_obj;
+  if (c->active_count() == 0) return PDES_OK;
+  const dim3 grid((unsigned)c->active_count());
   if (j->fp64)
-    hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, double>), dim3(c->numEl), dim3(256), 0, c->stream, c->numEl,
+    hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, double>), grid, dim3(256), 0, c->stream, c->active_list(),
                        j->d_rowptr, j->d_colind, (const double*)j->d_vals, p->d_lu, p->d_piv, j->d_err);
   else
-    hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, float>), dim3(c->numEl), dim3(256), 0, c->stream, c->numEl,
+    hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, float>), grid, dim3(256), 0, c->stream, c->active_list(),
                        j->d_rowptr, j->d_colind, (const float*)j->d_vals, p->d_lu, p->d_piv, j->d_err);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
@@ -159,8 +161,9 @@ static int bj_factor(pdes_ctx* c) {
 
 template <int NP, int ND>
 static int bj_apply(pdes_ctx* c, const double* b, double* x) {
-  hipLaunchKernelGGL((bjacobi_apply_kernel<NP, ND>), dim3(c->numEl), dim3(64), 0, c->stream, c->numEl,
-                     c->pc_obj->d_lu, c->pc_obj->d_piv, b, x);
+  if (c->active_count() == 0) return PDES_OK;
+  hipLaunchKernelGGL((bjacobi_apply_kernel<NP, ND>), dim3((unsigned)c->active_count()), dim3(64), 0, c->stream,
+                     c->active_list(), c->pc_obj->d_lu, c->pc_obj->d_piv, b, x);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -392,7 +395,8 @@ static int norm2_dev(pdes_ctx* c, GmresWork& W, const double* x, int off) {
 
 static int apply_op(pdes_ctx* c, int mode, const double* x, double* y) {
   if (mode == 0) return launch_spmv(c, x, y, 1.0, nullptr, 0.0);
-  return launch_residual(c, c->d_q, x, nullptr, y, false);
+  const int st = launch_residual(c, c->d_q, x, nullptr, y, false);
+  return st ? st : launch_ptc_shift(c, x, y);
 }
 
 // GMRES(restart), right preconditioned, x0 = x on entry (device vectors).
@@ -543,6 +547,7 @@ int pdes_gmres(pdes_ctx* c, const double* b_host, double* x_host, double rtol, d
 
 int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report* rep) {
   PDES_CHECK(c && o && rep, PDES_ERR_ARG, "null argument");
+  PDES_CHECK(!o->globalize_euler || o->euler_tau > 0, PDES_ERR_ARG, "euler_tau must be positive");
   auto t0 = std::chrono::steady_clock::now();
   *rep = pdes_newton_report{};
   const int64_t n = c->numDof;
@@ -565,13 +570,24 @@ int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report
     *out = std::sqrt(v);
     return PDES_OK;
   };
-  double nrm = 0;
+  double nrm = 0, tau = 0, prev_nrm = 0;
   if ((st = norm_res(&nrm)) != 0) goto fin;
   rep->res_norm0 = nrm;
   rep->res_norm = nrm;
   if (nrm < o->res_abstol) { rep->converged = 1; goto fin; }
   if ((st = jacobian_setup(c, o->fp64_blocks)) != 0) goto fin;
   for (int it = 1; it <= o->itermax; ++it) {
+    if (o->globalize_euler) {
+      // updateEuler at each Jacobian evaluation (globalization.jl:237-262)
+      if (it == 1) {
+        tau = o->euler_tau;
+        if (o->scale_euler) tau *= 1.0 / nrm;
+      } else {
+        tau *= prev_nrm / nrm;
+      }
+      prev_nrm = nrm;
+      if ((st = set_ptc_scale(c, &tau, 1)) != 0) goto fin;
+    }
     // doRecalculation: Jacobian and preconditioner every iteration (recalc_freq 1)
     if ((st = assemble(c)) != 0) goto fin;
     if ((st = pdes_pc_setup(c, o->pc_kind)) != 0) goto fin;
@@ -598,6 +614,7 @@ int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report
     }
   }
 fin:
+  c->ptc_on = false;
   hipFree(dq);
   hipFree(rhs);
   rep->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -50,6 +50,13 @@ _SIGS = {
                    _i32p, _dp],
     "pdes_newton_solve": [_vp, C.POINTER(abi.NewtonOpts), C.POINTER(abi.NewtonReport)],
     "pdes_eval_functional": [_vp, C.c_int, _vp, C.c_int, _dp],
+    "pdes_batch_create": [C.c_int, _vp, _vp, _vp, C.POINTER(_vp)],
+    "pdes_batch_destroy": [_vp],
+    "pdes_batch_num_groups": [_vp, C.POINTER(C.c_int)],
+    "pdes_batch_set_state": [_vp, C.c_int, _vp],
+    "pdes_batch_get_state": [_vp, C.c_int, _vp],
+    "pdes_batch_newton_solve": [_vp, C.POINTER(abi.NewtonOpts), _vp],
+    "pdes_batch_eval_residual": [_vp, C.c_int, _vp],
 }
 
 

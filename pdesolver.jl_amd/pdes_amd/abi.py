@@ -56,7 +56,8 @@ class NewtonOpts(C.Structure):
     _fields_ = [("res_abstol", C.c_double), ("res_reltol", C.c_double), ("step_tol", C.c_double),
                 ("itermax", C.c_int32), ("krylov_reltol", C.c_double), ("krylov_abstol", C.c_double),
                 ("krylov_itermax", C.c_int32), ("gmres_restart", C.c_int32), ("pc_kind", C.c_int32),
-                ("fp64_blocks", C.c_int32), ("jv_mode", C.c_int32)]
+                ("fp64_blocks", C.c_int32), ("jv_mode", C.c_int32), ("globalize_euler", C.c_int32),
+                ("euler_tau", C.c_double), ("scale_euler", C.c_int32)]
 
 
 class NewtonReport(C.Structure):

@@ -196,11 +196,17 @@ class NewtonOptions:
     pc_kind: int = abi.PDES_PC_BJACOBI
     fp64_blocks: bool = False
     jv_mode: int = 0
+    # implicit-Euler globalization (newton_globalize_euler, euler_tau,
+    # newton_scale_euler; read_input.jl:421-422)
+    globalize_euler: bool = False
+    euler_tau: float = 1.0
+    scale_euler: bool = False
 
     def desc(self):
         return abi.NewtonOpts(self.res_abstol, self.res_reltol, self.step_tol, self.itermax,
                               self.krylov_reltol, self.krylov_abstol, self.krylov_itermax,
-                              self.gmres_restart, self.pc_kind, int(self.fp64_blocks), self.jv_mode)
+                              self.gmres_restart, self.pc_kind, int(self.fp64_blocks), self.jv_mode,
+                              int(self.globalize_euler), self.euler_tau, int(self.scale_euler))
 
 
 def newton(mesh, sbp, eqn: EulerData, opts: NewtonOptions | None = None):

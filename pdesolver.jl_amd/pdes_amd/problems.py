@@ -116,3 +116,28 @@ def make_problem(kind: str, n: int, p: int, op_kind: str = "omega", flux: str = 
     else:
         raise ValueError(flux)
     return m, op, phys
+
+
+def config3_instances(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, amp: float = 1e-3):
+    """Batched uneven solves: `count` 2D vortex problems with the grid edge
+    count uniform in nmin..nmax, operator Omega or diagonal-E and inflow Mach
+    uniform in [0.3, 0.95]; state = exact * (1 + U(-amp, amp)).
+
+    Omega instances use the weak form with the Roe face flux and p in {1,2,3};
+    diagonal-E instances use the entropy-stable pairing the reference runs
+    steady diagonal-E cases with (volume_integral_type 2, IRSLF faces; e.g.
+    test/euler/test_jacp.jl:66-95) and p in {2,3}: at p = 1 that
+    discretisation has no Newton-reachable steady state on these grids (the
+    oracle's direct Newton stalls near 1e-3 with or without the implicit-Euler
+    globalization).  Returns a list of (mesh, op, phys, q0)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        kind = "omega" if rng.integers(0, 2) == 0 else "diage"
+        p = int(rng.integers(1, 4)) if kind == "omega" else int(rng.integers(2, 4))
+        Min = float(rng.uniform(0.3, 0.95))
+        m, op, ph = make_problem("vortex", n, p, kind, "roe" if kind == "omega" else "ir", vortex_Min=Min)
+        q0 = vortex_noise_state(m, ph, seed=seed * 1000 + i, amp=amp)
+        out.append((m, op, ph, q0))
+    return out

@@ -1,0 +1,91 @@
+"""Batched steady solves (config 3 shape, small sizes) and their isolation
+guarantees, through the C ABI."""
+import numpy as np
+import pytest
+
+from pdes_amd import euler, linsolve, problems
+from pdes_amd.batch import BatchSolver
+
+pytestmark = pytest.mark.gpu
+
+OPTS = linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-4, itermax=20)
+
+
+def small_instances(count=7, seed=11):
+    return problems.config3_instances(count=count, seed=seed, nmin=3, nmax=9)
+
+
+def test_packed_residual_parity(oracle):
+    """The packed residual (per-instance inflow Mach in the boundary states)
+    equals the oracle residual of each instance."""
+    from pdes_amd import abi
+    from helpers import assert_parity
+    inst = small_instances()
+    bs = BatchSolver([(m, op, ph) for (m, op, ph, q0) in inst])
+    for i, (m, op, ph, q0) in enumerate(inst):
+        bs.set_state(i, q0)
+    for i, (m, op, ph, q0) in enumerate(inst):
+        b = abi.make_descriptors(m, op, ph)
+        r = bs.eval_residual(i)
+        print(i, op.kind, op.degree, m.numEl, "Min", ph.vortex_Min, flush=True)
+        assert_parity(r, oracle.residual(b, q0), oracle.residual_scale(b, q0), 1e-12, f"instance {i}")
+    bs.free()
+
+
+def test_batch_converges_and_matches_single_context():
+    inst = small_instances()
+    bs = BatchSolver([(m, op, ph) for (m, op, ph, q0) in inst])
+    print("groups", bs.num_groups(), flush=True)
+    for i, (m, op, ph, q0) in enumerate(inst):
+        bs.set_state(i, q0)
+    reps = bs.solve(OPTS)
+    for i, (m, op, ph, q0) in enumerate(inst):
+        r = reps[i]
+        print(i, op.kind, op.degree, m.numEl, "it", r.iterations, "norm", r.res_norm, flush=True)
+        assert r.converged == 1 and r.res_norm < 1e-10
+        # same problem through the single-context Newton path
+        eqn = euler.EulerData(m, op, ph)
+        eqn.set_q(q0)
+        rep1 = linsolve.newton(m, op, eqn, OPTS)
+        assert rep1.converged
+        q1 = eqn.get_q()
+        qb = bs.get_state(i)
+        assert np.abs(qb - q1).max() <= 1e-9 * np.abs(q1).max()
+        eqn.free()
+    bs.free()
+
+
+def test_batch_result_independent_of_mix():
+    """Each instance gives the same bits alone and inside the mixed batch."""
+    inst = small_instances(count=6, seed=5)
+    bs = BatchSolver([(m, op, ph) for (m, op, ph, q0) in inst])
+    for i, (m, op, ph, q0) in enumerate(inst):
+        bs.set_state(i, q0)
+    reps = bs.solve(OPTS)
+    mixed = [bs.get_state(i) for i in range(len(inst))]
+    bs.free()
+    for i, (m, op, ph, q0) in enumerate(inst):
+        alone = BatchSolver([(m, op, ph)])
+        alone.set_state(0, q0)
+        r = alone.solve(OPTS)[0]
+        assert r.iterations == reps[i].iterations
+        assert r.total_krylov_iters == reps[i].total_krylov_iters
+        assert np.array_equal(alone.get_state(0), mixed[i]), f"instance {i} differs"
+        alone.free()
+
+
+def test_batch_failure_isolated():
+    """checkDensity/checkPressure failing in one instance retires it (-1)
+    without stopping the others."""
+    inst = small_instances(count=4, seed=9)
+    bs = BatchSolver([(m, op, ph) for (m, op, ph, q0) in inst])
+    for i, (m, op, ph, q0) in enumerate(inst):
+        q = q0.copy()
+        if i == 2:
+            q[0, 0, 0] = -1.0
+        bs.set_state(i, q)
+    reps = bs.solve(OPTS)
+    assert reps[2].converged == -1
+    for i in (0, 1, 3):
+        assert reps[i].converged == 1 and reps[i].res_norm < 1e-10
+    bs.free()

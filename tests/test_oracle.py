@@ -352,3 +352,29 @@ def test_complex_step_jacobian_matches_jv(oracle):
         v = np.random.default_rng(1).standard_normal(q.size)
         jv = oracle.jv_complex_step(b, q, v.reshape(q.shape)).ravel()
         assert np.abs(J @ v - jv).max() < 1e-11 * (1 + np.abs(jv).max())
+
+
+def test_implicit_euler_globalization_limits(oracle):
+    """The globalized Newton step solves (J - M/tau_vec) dq = -R.  For a tiny
+    tau it is the explicit pseudo-time step dq = tau_vec M^-1 R with
+    tau_vec = tau/(1 + sqrt(jac)) per node (calcTauVec,
+    globalization.jl:181-201), and a converged globalized solve reaches the
+    same steady state as plain Newton."""
+    m, op, ph, b = setup("vortex", 4, 2, "omega", "roe")
+    q0 = problems.vortex_noise_state(m, ph, seed=4)
+    minv = np.repeat((1.0 / (op.w[None, :] / m.jac))[:, :, None], m.dim + 2, axis=2)
+    tau = 1e-7
+    q1, _ = oracle.newton_direct(b, q0, minv, res_abstol=0.0, itermax=1, euler_tau=tau, jac=m.jac)
+    r = oracle.residual(b, q0)
+    tau_vec = (tau / (1.0 + np.sqrt(m.jac)))[:, :, None]
+    expect = tau_vec * minv * r
+    assert np.abs((q1 - q0) - expect).max() <= 1e-5 * np.abs(expect).max()
+    # scale_euler: the first tau is euler_tau / ||R_0||
+    nrm0 = float(np.sqrt((r * r * minv).sum()))
+    q2, _ = oracle.newton_direct(b, q0, minv, res_abstol=0.0, itermax=1, euler_tau=tau * nrm0, jac=m.jac,
+                                 scale_euler=True)
+    assert np.abs((q2 - q0) - expect).max() <= 1e-5 * np.abs(expect).max()
+    qa, na = oracle.newton_direct(b, q0, minv, res_abstol=1e-11)
+    qb, nb = oracle.newton_direct(b, q0, minv, res_abstol=1e-11, euler_tau=1.0, jac=m.jac)
+    assert na[-1] < 1e-11 and nb[-1] < 1e-11 and len(nb) >= len(na)
+    assert np.abs(qa - qb).max() <= 1e-10 * np.abs(qa).max()
