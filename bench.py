@@ -30,13 +30,58 @@ FP64_PEAK_TFLOPS = 78.6
 
 
 def residual_bytes_per_launch(m, op):
-    """Algorithmic HBM bytes of one residual launch: q read + res write +
-    dxidx read (per node), the element->face records, the face normals (once
-    per interface) and the boundary face data, each touched once."""
+    """Algorithmic HBM bytes of one residual launch, each array touched once:
+    q read + res write (ndof doubles per node), the metric dxidx (per node, or
+    once per element when every node of the element has the same metric, as on
+    these straight-sided triangles), the element->face records (16 B per face
+    slot), the face normals once per interface and the boundary normals and
+    coordinates once per boundary face.  SBP tables are excluded (cache
+    resident)."""
     Np, Nf, dim, nd = op.numnodes, op.face.numnodes, m.dim, m.dim + 2
-    per_el = 8 * (nd * Np + nd * Np + dim * dim * Np) + 16 * (dim + 1)
+    affine = bool(np.all(m.dxidx == m.dxidx[:, :1]))
+    metric = 8 * dim * dim * (1 if affine else Np)
+    per_el = 8 * (nd * Np + nd * Np) + metric + 16 * (dim + 1)
     faces = 8 * dim * Nf * m.numInterfaces + 8 * 2 * dim * Nf * m.numBoundaryFaces
     return per_el * m.numEl + faces
+
+
+# Algorithmic FP64 flops of the 2D Ismail-Roe split-form residual, counted from
+# the kernel's arithmetic with fma = 2 and div = sqrt = log = pow = 1:
+#   two-point flux F*(q_j, q_k) in both parametric directions   71
+#     (two logavg on the series branch 2 x 14, Roe-like averages 15,
+#      two directions 2 x 11)
+#   res_j -= 2 S_jk F*, res_k += 2 S_jk F* for 4 variables x 2 directions  32
+# per node: IR z-variables 15; per face node: IRSLF flux (two z-variable sets,
+# single-direction IR flux, two entropy-variable conversions, A0, lambda_max,
+# lambda A0 dv) 230 plus the two-sided face integration 16.
+IR_FLOPS_PER_PAIR = 71 + 32
+IR_FLOPS_PER_NODE = 15
+IRSLF_FLOPS_PER_FACE_NODE = 230 + 16
+
+
+def residual_flops_per_launch(m, op):
+    """Algorithmic flops of one split-form (IR + IRSLF) residual launch."""
+    Np, Nf = op.numnodes, op.face.numnodes
+    pairs = Np * (Np - 1) // 2
+    face_nodes = Nf * (m.numInterfaces + m.numBoundaryFaces)
+    return (pairs * IR_FLOPS_PER_PAIR + Np * IR_FLOPS_PER_NODE) * m.numEl + face_nodes * IRSLF_FLOPS_PER_FACE_NODE
+
+
+def traffic_for(p, flux):
+    """Measured HBM bytes per launch of the residual kernel for this variant,
+    read from the committed counter summary (FETCH_SIZE + WRITE_SIZE per
+    dispatch, scaled by the same counters on a copy kernel of known size);
+    None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "euler_residual_256x256_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    key = f"| p{p}_{flux} |"
+    with open(path) as f:
+        for line in f:
+            if line.startswith(key):
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                return float(cells[-1])
+    return None
 
 
 def variants():
@@ -95,7 +140,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run one 512 MiB device-to-device copy (torch) to calibrate HBM byte counters")
+                    help="few copy-kernel launches only (for counter runs)")
     ap.add_argument("--variants", default="all", help="comma list like p2_roe,p4_ir (default all)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -127,14 +172,30 @@ def main():
         # residual kernel alone, HIP events on the context stream
         kms = euler.time_residual(eqn, 50)
         nbytes = residual_bytes_per_launch(m, op)
-        kern[(p, flux)] = (kms, nbytes)
-        results[f"p{p}_{flux}"] = {"dof_evals_per_s": dof_evals / (ms / 1e3), "ms_per_rk4_step": ms / args.steps,
-                                   "residual_kernel_ms": kms, "residual_GBs": nbytes / (kms / 1e3) / 1e9,
-                                   "numDof": int(q.size), "launch_variants": tuned}
+        nflops = residual_flops_per_launch(m, op) if flux == "ir" else None
+        kern[(p, flux)] = (kms, nbytes, nflops)
+        r = {"dof_evals_per_s": dof_evals / (ms / 1e3), "ms_per_rk4_step": ms / args.steps,
+             "residual_kernel_ms": kms, "residual_alg_bytes": nbytes,
+             "residual_GBs": nbytes / (kms / 1e3) / 1e9, "numDof": int(q.size), "launch_variants": tuned}
+        if nflops is not None:
+            r["residual_alg_flops"] = nflops
+            r["residual_TFLOPs"] = nflops / (kms / 1e3) / 1e12
+        results[f"p{p}_{flux}"] = r
         eqn.free()
-    # dominant kernel by time in the step
-    (pd, fd), (kms, nbytes) = max(kern.items(), key=lambda kv: kv[1][0])
-    achieved = nbytes / (kms / 1e3) / 1e9
+    # dominant kernel by time in the step: the split form is FP64-bound (priced
+    # in flops against the FP64 peak), the weak form HBM-bound (priced in bytes)
+    (pd, fd), (kms, nbytes, nflops) = max(kern.items(), key=lambda kv: kv[1][0])
+    if nflops is not None:
+        achieved = nflops / (kms / 1e3) / 1e12
+        roof = {"kernel": f"euler_residual p={pd} {fd} split form", "bound": "mfma", "achieved": achieved,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                "pipe": "FP64 vector ALU (same 78.6 TFLOP/s peak as FP64 MFMA)",
+                "traffic": traffic_for(pd, fd), "alg_bytes": nbytes}
+    else:
+        achieved = nbytes / (kms / 1e3) / 1e9
+        roof = {"kernel": f"euler_residual p={pd} {fd} weak form", "bound": "hbm", "achieved": achieved,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic_for(pd, fd)}
     line = {
         "metric": "DOF-residual evals/s",
         "value": total_dof_evals / (total_ms / 1e3),
@@ -152,15 +213,14 @@ def main():
                                "SBP diagonal-E p=2 and p=4, Roe weak form and Ismail-Roe split form, "
                                "residuals inside RK4 stages",
                    "variants": results},
-        "roofline": {"kernel": f"euler_residual p={pd} {fd}", "bound": "hbm", "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "roofline": roof,
     }
-    if args.calibrate_copy:
-        import torch
-        x = torch.empty(512 * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
+    # plain copy kernel, 1 GiB read + 1 GiB write: attainable bandwidth, and the
+    # reference point that scales the HBM byte counters of the residual kernel
+    copy_bytes = 1 << 30
+    cms = euler.copy_bandwidth(copy_bytes, 3 if args.calibrate_copy else 20)
+    line["copy_reference"] = {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "kernel_ms": cms,
+                              "GBs": 2 * copy_bytes / (cms / 1e3) / 1e9}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

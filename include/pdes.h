@@ -140,6 +140,11 @@ int pdes_set_launch_variant(pdes_ctx* ctx, int plain_variant, int rk_variant);
  * state with 4 residual evaluations per step, h the time step; elapsed_ms (if
  * not null) receives the HIP-event time of all steps. */
 int pdes_time_residual(pdes_ctx* ctx, int iters, double* avg_ms);
+/* Measurement helper with no counterpart in the reference: copies nbytes of
+ * doubles between two fresh device buffers with a plain grid-stride kernel
+ * (pdes_copy_kernel) `iters` times; avg_ms receives the average kernel time.
+ * Used as the attainable-bandwidth reference and to scale HBM byte counters. */
+int pdes_copy_bandwidth(int64_t nbytes, int iters, double* avg_ms);
 int pdes_rk4(pdes_ctx* ctx, int nsteps, double h, double* elapsed_ms);
 int pdes_jv_device(pdes_ctx* ctx, const double* v_dev, double* b_dev);
 int pdes_sync(pdes_ctx* ctx);

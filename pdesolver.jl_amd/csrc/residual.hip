@@ -52,6 +52,12 @@ __global__ void wnorm2_partial(const double* __restrict__ x, const double* __res
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// y = x over n2 double2 elements, grid-stride (the copy reference kernel)
+__global__ void pdes_copy_kernel(const double2* __restrict__ x, double2* __restrict__ y, int64_t n2) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += stride) y[i] = x[i];
+}
+
 __global__ void sum_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
   __shared__ double sh[RED_THREADS];
   double acc = 0.0;
@@ -155,6 +161,38 @@ int pdes_set_launch_variant(pdes_ctx* c, int plain_variant, int rk_variant) {
 
 // Times `iters` back-to-back residual evaluations on the context stream with
 // HIP events; *avg_ms is the average per evaluation (one kernel launch each).
+int pdes_copy_bandwidth(int64_t nbytes, int iters, double* avg_ms) {
+  PDES_CHECK(avg_ms && iters > 0 && nbytes >= 16 && nbytes % 16 == 0, PDES_ERR_ARG, "bad argument");
+  const int64_t n2 = nbytes / 16;
+  double2 *x = nullptr, *y = nullptr;
+  if (hipMalloc((void**)&x, nbytes) != hipSuccess) { set_error("copy buffer"); return PDES_ERR_NOMEM; }
+  if (hipMalloc((void**)&y, nbytes) != hipSuccess) { hipFree(x); set_error("copy buffer"); return PDES_ERR_NOMEM; }
+  hipEvent_t a = nullptr, b = nullptr;
+  float ms = 0;
+  const unsigned grid = (unsigned)std::min<int64_t>(8192, (n2 + 255) / 256);
+  int st = PDES_OK;
+  if (hipMemset(x, 0, nbytes) != hipSuccess || hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    set_error("copy setup");
+    st = PDES_ERR_HIP;
+  }
+  if (!st) {
+    hipLaunchKernelGGL(pdes_copy_kernel, dim3(grid), dim3(256), 0, nullptr, x, y, n2);  // warm
+    hipEventRecord(a, nullptr);
+    for (int it = 0; it < iters; ++it) hipLaunchKernelGGL(pdes_copy_kernel, dim3(grid), dim3(256), 0, nullptr, x, y, n2);
+    hipEventRecord(b, nullptr);
+    if (hipEventSynchronize(b) != hipSuccess || hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+      set_error("copy timing");
+      st = PDES_ERR_HIP;
+    }
+  }
+  if (a) hipEventDestroy(a);
+  if (b) hipEventDestroy(b);
+  hipFree(x);
+  hipFree(y);
+  *avg_ms = ms / iters;
+  return st;
+}
+
 int pdes_time_residual(pdes_ctx* c, int iters, double* avg_ms) {
   PDES_CHECK(c && avg_ms && iters > 0, PDES_ERR_ARG, "bad argument");
   hipEvent_t a, b;

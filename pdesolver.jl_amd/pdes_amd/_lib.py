@@ -35,6 +35,7 @@ _SIGS = {
     "pdes_jv_complex_step": [_vp, _vp, _vp],
     "pdes_jv_device": [_vp, _vp, _vp],
     "pdes_time_residual": [_vp, C.c_int, _dp],
+    "pdes_copy_bandwidth": [C.c_int64, C.c_int, _dp],
     "pdes_autotune": [_vp, C.c_int, _i32p],
     "pdes_set_launch_variant": [_vp, C.c_int, C.c_int],
     "pdes_rk4": [_vp, C.c_int, C.c_double, _dp],

@@ -130,6 +130,14 @@ def autotune(eqn: EulerData, iters: int = 10):
     return int(out[0]), int(out[1])
 
 
+def copy_bandwidth(nbytes: int = 1 << 30, iters: int = 20) -> float:
+    """Average duration (ms) of a plain device copy of nbytes (read nbytes,
+    write nbytes) over `iters` launches."""
+    ms = C.c_double(0.0)
+    check(lib().pdes_copy_bandwidth(int(nbytes), int(iters), C.byref(ms)))
+    return ms.value
+
+
 def time_residual(eqn: EulerData, iters: int) -> float:
     """Average duration (ms) of the residual kernel over `iters` launches."""
     ms = C.c_double(0.0)
