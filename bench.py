@@ -171,12 +171,20 @@ def main():
         total_ms += ms
         # residual kernel alone, HIP events on the context stream
         kms = euler.time_residual(eqn, 50)
+        # the same evaluation through the host-buffer boundary: q uploaded from
+        # host memory, residual, res copied back (PCIe-inclusive; not `value`)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            eqn.set_q(q)
+            euler.evalResidual(m, op, eqn)
+        host_ms = (time.perf_counter() - t0) / 5 * 1e3
         nbytes = residual_bytes_per_launch(m, op)
         nflops = residual_flops_per_launch(m, op) if flux == "ir" else None
         kern[(p, flux)] = (kms, nbytes, nflops)
         r = {"dof_evals_per_s": dof_evals / (ms / 1e3), "ms_per_rk4_step": ms / args.steps,
              "residual_kernel_ms": kms, "residual_alg_bytes": nbytes,
-             "residual_GBs": nbytes / (kms / 1e3) / 1e9, "numDof": int(q.size), "launch_variants": tuned}
+             "residual_GBs": nbytes / (kms / 1e3) / 1e9, "numDof": int(q.size), "launch_variants": tuned,
+             "host_boundary_ms": host_ms, "host_boundary_dof_evals_per_s": q.size / (host_ms / 1e3)}
         if nflops is not None:
             r["residual_alg_flops"] = nflops
             r["residual_TFLOPs"] = nflops / (kms / 1e3) / 1e12

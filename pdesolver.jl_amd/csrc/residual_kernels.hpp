@@ -24,7 +24,7 @@ namespace pdes {
 //   x: neighbour element (>= 0) or -1 for a boundary face
 //   y: interface index (interior) or boundary face index
 //   z: neighbour face | orient << 8 | side << 16  (side 0: this element is elementL)
-//   w: BC functor of a boundary face (PDES_BC_*)
+//   w: BC functor of a boundary face (PDES_BC_*) | instance << 8 (packed batches)
 struct ElFace { int32_t x, y, z, w; };
 
 struct PhysConst;
