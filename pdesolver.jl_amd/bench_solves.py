@@ -1,0 +1,128 @@
+"""Batched steady solves: solves/s of many independent uneven 2D vortex
+problems, packed into one batch, against the plain one-solve-at-a-time device
+path on the same problems.
+
+Workload: 96 instances (seed 3), grid edge count uniform in 8..128, Omega p in
+{1,2,3} with the Roe weak form or diagonal-E p in {2,3} with the IR split form
+and IRSLF faces, inflow Mach uniform in [0.3, 0.95], exact state times
+(1 + U(-1e-3, 1e-3)).  Each solve is Newton to ||R||_{M^-1} <= 1e-10 with FP32
+Jacobian blocks and FP32 block-Jacobi under FP64 GMRES.
+
+Both paths build their device objects before the timed region; the timed
+region is state upload + Newton solve (+ nothing else) for every instance.
+
+    python pdesolver.jl_amd/bench_solves.py [--count 96] [--nmax 128]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+from pdes_amd import euler, linsolve, problems  # noqa: E402
+from pdes_amd.batch import BatchSolver  # noqa: E402
+
+
+def newton_options():
+    return linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=50)
+
+
+def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_tol: float = 1e-8,
+        only: str = "both"):
+    t0 = time.perf_counter()
+    inst = problems.config3_instances(count=count, seed=seed, nmin=nmin, nmax=nmax)
+    t_gen = time.perf_counter() - t0
+    opts = newton_options()
+    ndof = [int(q.size) for (m, op, ph, q) in inst]
+
+    if only == "batched":
+        bs = BatchSolver([(m, op, ph) for (m, op, ph, q) in inst])
+        t0 = time.perf_counter()
+        for i, (m, op, ph, q0) in enumerate(inst):
+            bs.set_state(i, q0)
+        reps = bs.solve(opts)
+        t_batch = time.perf_counter() - t0
+        bs.free()
+        return {"instances": count, "batched_s": t_batch, "batched_solves_per_s": count / t_batch,
+                "converged_batched": sum(1 for r in reps if r.converged == 1),
+                "krylov_iters_batched": int(sum(r.total_krylov_iters for r in reps))}
+    # one solve at a time: a context per instance, created up front
+    ctxs = [euler.EulerData(m, op, ph) for (m, op, ph, q) in inst]
+    seq_lat = []
+    seq_reps = []
+    t0 = time.perf_counter()
+    for e, (m, op, ph, q0) in zip(ctxs, inst):
+        t1 = time.perf_counter()
+        e.set_q(q0)
+        seq_reps.append(linsolve.newton(m, op, e, opts))
+        seq_lat.append(time.perf_counter() - t1)
+        if len(seq_lat) % 16 == 0:
+            print(f"sequential {len(seq_lat)}/{count} solved", file=sys.stderr, flush=True)
+    t_seq = time.perf_counter() - t0
+    q_seq = [e.get_q() for e in ctxs]
+    for e in ctxs:
+        e.free()
+
+    # batched
+    print("batched solve", file=sys.stderr, flush=True)
+    bs = BatchSolver([(m, op, ph) for (m, op, ph, q) in inst])
+    t0 = time.perf_counter()
+    for i, (m, op, ph, q0) in enumerate(inst):
+        bs.set_state(i, q0)
+    reps = bs.solve(opts)
+    t_batch = time.perf_counter() - t0
+    q_bat = [bs.get_state(i) for i in range(len(inst))]
+    groups = bs.num_groups()
+    bs.free()
+
+    worst = 0.0
+    for a, b in zip(q_seq, q_bat):
+        worst = max(worst, float(np.abs(a - b).max() / np.abs(a).max()))
+    conv_seq = sum(1 for r in seq_reps if r.converged == 1 and r.res_norm <= 1e-10)
+    conv_bat = sum(1 for r in reps if r.converged == 1 and r.res_norm <= 1e-10)
+    return {
+        "workload": f"{count} 2D vortex instances (seed {seed}), n in {nmin}..{nmax}, Omega p1-3 Roe / "
+                    "diagonal-E p2-3 IR split, Min in [0.3,0.95], Newton to ||R|| <= 1e-10",
+        "instances": count,
+        "total_dofs": int(sum(ndof)),
+        "groups": groups,
+        "batched_solves_per_s": count / t_batch,
+        "sequential_solves_per_s": count / t_seq,
+        "speedup": t_seq / t_batch,
+        "batched_s": t_batch,
+        "sequential_s": t_seq,
+        "sequential_latency_p50_s": float(np.percentile(seq_lat, 50)),
+        "sequential_latency_p99_s": float(np.percentile(seq_lat, 99)),
+        "converged_batched": conv_bat,
+        "converged_sequential": conv_seq,
+        "newton_iters_batched": int(sum(r.iterations for r in reps)),
+        "newton_iters_sequential": int(sum(r.iterations for r in seq_reps)),
+        "krylov_iters_batched": int(sum(r.total_krylov_iters for r in reps)),
+        "krylov_iters_sequential": int(sum(r.total_krylov_iters for r in seq_reps)),
+        "max_rel_state_diff": worst,
+        "states_agree": worst <= check_tol,
+        "setup_s": t_gen,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--count", type=int, default=96)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--nmin", type=int, default=8)
+    ap.add_argument("--nmax", type=int, default=128)
+    ap.add_argument("--only", choices=["both", "batched"], default="both")
+    a = ap.parse_args()
+    print(json.dumps(run(a.count, a.seed, a.nmin, a.nmax, only=a.only)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

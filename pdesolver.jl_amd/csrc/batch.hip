@@ -42,9 +42,11 @@ __device__ __forceinline__ int64_t active_dof(const int32_t* el_list, int64_t t,
 __global__ void __launch_bounds__(SEG_THREADS)
 seg_dot_partial(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ w,
                 const double* __restrict__ wt, const int64_t* __restrict__ cbeg, const int32_t* __restrict__ clen,
-                const int32_t* __restrict__ clist, double* __restrict__ partial, int mstride) {
+                const int32_t* __restrict__ clist, double* __restrict__ partial, int mstride,
+                const int32_t* __restrict__ chunk_inst, const int32_t* __restrict__ mask) {
   __shared__ double sh[SEG_THREADS];
   const int c = clist[blockIdx.x];
+  if (mask && !mask[chunk_inst[c]]) return;
   const int64_t b0 = cbeg[c];
   const int len = clen[c];
   for (int i = 0; i < m; ++i) {
@@ -68,10 +70,10 @@ seg_dot_partial(const double* __restrict__ V, int m, int64_t ldv, const double* 
 // out[inst][i] = sum of the instance's chunk partials in chunk order
 __global__ void seg_dot_reduce(const double* __restrict__ partial, int m, int mstride,
                                const int32_t* __restrict__ inst_c0, const int32_t* __restrict__ ilist,
-                               double* __restrict__ out) {
+                               double* __restrict__ out, const int32_t* __restrict__ mask) {
   const int inst = ilist[blockIdx.x];
   const int i = threadIdx.x;
-  if (i >= m) return;
+  if (i >= m || (mask && !mask[inst])) return;
   double acc = 0.0;
   for (int c = inst_c0[inst]; c < inst_c0[inst + 1]; ++c) acc += partial[(int64_t)c * mstride + i];
   out[(int64_t)inst * mstride + i] = acc;
@@ -80,11 +82,14 @@ __global__ void seg_dot_reduce(const double* __restrict__ partial, int m, int ms
 // w -= sum_i coef[inst][i] V_i   over the active dofs
 __global__ void seg_multiaxpy(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ coef,
                               int mstride, double* __restrict__ w, const int32_t* __restrict__ el_list,
-                              const int32_t* __restrict__ el_inst, int dpe, int64_t n) {
+                              const int32_t* __restrict__ el_inst, int dpe, int64_t n,
+                              const int32_t* __restrict__ mask) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
     const int64_t k = active_dof(el_list, t, dpe);
-    const double* cf = coef + (int64_t)el_inst[k / dpe] * mstride;
+    const int inst = el_inst[k / dpe];
+    if (mask && !mask[inst]) continue;
+    const double* cf = coef + (int64_t)inst * mstride;
     double acc = w[k];
     for (int i = 0; i < m; ++i) acc -= cf[i] * V[(int64_t)i * ldv + k];
     w[k] = acc;
@@ -94,11 +99,14 @@ __global__ void seg_multiaxpy(const double* __restrict__ V, int m, int64_t ldv, 
 // y = sum_i coef[inst][i] V_i  over the active dofs
 __global__ void seg_combine(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ coef,
                             int mstride, double* __restrict__ y, const int32_t* __restrict__ el_list,
-                            const int32_t* __restrict__ el_inst, int dpe, int64_t n) {
+                            const int32_t* __restrict__ el_inst, int dpe, int64_t n,
+                            const int32_t* __restrict__ mask) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
     const int64_t k = active_dof(el_list, t, dpe);
-    const double* cf = coef + (int64_t)el_inst[k / dpe] * mstride;
+    const int inst = el_inst[k / dpe];
+    if (mask && !mask[inst]) continue;
+    const double* cf = coef + (int64_t)inst * mstride;
     double acc = 0.0;
     // basis vectors past an instance's own Krylov dimension are stale: skip them
     for (int i = 0; i < m; ++i)
@@ -110,12 +118,175 @@ __global__ void seg_combine(const double* __restrict__ V, int m, int64_t ldv, co
 // y = a[inst] * x (+ y)   over the active dofs
 __global__ void seg_scale(const double* __restrict__ x, double* __restrict__ y, const double* __restrict__ a,
                           int astride, int accumulate, const int32_t* __restrict__ el_list,
-                          const int32_t* __restrict__ el_inst, int dpe, int64_t n) {
+                          const int32_t* __restrict__ el_inst, int dpe, int64_t n,
+                          const int32_t* __restrict__ mask) {
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
     const int64_t k = active_dof(el_list, t, dpe);
-    const double s = a[(int64_t)el_inst[k / dpe] * astride];
+    const int inst = el_inst[k / dpe];
+    if (mask && !mask[inst]) continue;
+    const double s = a[(int64_t)inst * astride];
     y[k] = accumulate ? y[k] + s * x[k] : s * x[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// device-resident GMRES control: one thread per listed instance
+// ---------------------------------------------------------------------------
+// per-instance GMRES scalars, [inst][GS_*]
+enum { GS_BETA0 = 0, GS_RES = 1, GS_NSCAL = 2 };
+
+// sums of the chunk partials -> coef[inst][i] (the CGS coefficients) and the
+// Hessenberg column j (pass 0 stores, pass 1 adds)
+__global__ void seg_reduce_hcol(const double* __restrict__ partial, int m, int mstride,
+                                const int32_t* __restrict__ inst_c0, const int32_t* __restrict__ ilist,
+                                const int32_t* __restrict__ mask, double* __restrict__ coef, double* __restrict__ H,
+                                int R, int j, int pass) {
+  const int inst = ilist[blockIdx.x];
+  const int i = threadIdx.x;
+  if (i >= m || !mask[inst]) return;
+  double acc = 0.0;
+  for (int c = inst_c0[inst]; c < inst_c0[inst + 1]; ++c) acc += partial[(int64_t)c * mstride + i];
+  coef[(int64_t)inst * mstride + i] = acc;
+  double* h = H + (int64_t)inst * (R + 1) * R + (int64_t)i * R + j;
+  *h = pass ? *h + acc : acc;
+}
+
+// Arnoldi step j done: ||w|| is in seg[inst][0]; apply the previous Givens
+// rotations to column j, form rotation j, update g and the residual estimate,
+// and decide whether the instance keeps iterating.
+__global__ void gmres_finish_step(int nlist, const int32_t* __restrict__ ilist, int32_t* __restrict__ iter,
+                                  int32_t* __restrict__ live, int32_t* __restrict__ total, int32_t* __restrict__ jdone,
+                                  const double* __restrict__ seg, double* __restrict__ coef, int mstride,
+                                  double* __restrict__ H, double* __restrict__ cs, double* __restrict__ sn,
+                                  double* __restrict__ g, double* __restrict__ gs, int R, int j, double rtol,
+                                  double atol, int maxit) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int inst = ilist[t];
+  if (!iter[inst]) return;
+  double* Hi = H + (int64_t)inst * (R + 1) * R;
+  double* ci = cs + (int64_t)inst * R;
+  double* si = sn + (int64_t)inst * R;
+  double* gi = g + (int64_t)inst * (R + 1);
+  const double hn = sqrt(seg[(int64_t)inst * mstride]);
+  Hi[(j + 1) * R + j] = hn;
+  for (int k = 0; k < j; ++k) {
+    const double a = Hi[k * R + j], b = Hi[(k + 1) * R + j];
+    Hi[k * R + j] = ci[k] * a + si[k] * b;
+    Hi[(k + 1) * R + j] = -si[k] * a + ci[k] * b;
+  }
+  const double a = Hi[j * R + j], b = Hi[(j + 1) * R + j];
+  const double r = hypot(a, b);
+  ci[j] = r > 0 ? a / r : 1.0;
+  si[j] = r > 0 ? b / r : 0.0;
+  Hi[j * R + j] = r;
+  Hi[(j + 1) * R + j] = 0.0;
+  gi[j + 1] = -si[j] * gi[j];
+  gi[j] = ci[j] * gi[j];
+  const double res = fabs(gi[j + 1]);
+  gs[(int64_t)inst * GS_NSCAL + GS_RES] = res;
+  total[inst] += 1;
+  jdone[inst] = j + 1;
+  const double tol = fmax(rtol * gs[(int64_t)inst * GS_NSCAL + GS_BETA0], atol);
+  const bool conv = res <= tol || hn == 0.0;
+  if (conv) live[inst] = 0;
+  if (conv || total[inst] >= maxit || j + 1 == R) iter[inst] = 0;
+  coef[(int64_t)inst * mstride] = hn > 0 ? 1.0 / hn : 0.0;
+}
+
+// end of a cycle: y = H^-1 g for the instances that took part -> coef[inst][0..R)
+__global__ void gmres_solve_y(int nlist, const int32_t* __restrict__ ilist, const int32_t* __restrict__ part,
+                              const int32_t* __restrict__ jdone, const double* __restrict__ H,
+                              const double* __restrict__ g, double* __restrict__ coef, int mstride, int R) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int inst = ilist[t];
+  double* y = coef + (int64_t)inst * mstride;
+  for (int k = 0; k < mstride; ++k) y[k] = 0.0;
+  if (!part[inst]) return;
+  const double* Hi = H + (int64_t)inst * (R + 1) * R;
+  const double* gi = g + (int64_t)inst * (R + 1);
+  const int n = jdone[inst];
+  for (int k = n - 1; k >= 0; --k) {
+    double v = gi[k];
+    for (int l = k + 1; l < n; ++l) v -= Hi[k * R + l] * y[l];
+    y[k] = v / Hi[k * R + k];
+  }
+}
+
+// (re)start of a cycle: beta = ||b - A x|| in seg[inst][0]; convergence and
+// iteration-limit tests; g = beta e_1; coef[inst][0] = 1/beta
+__global__ void gmres_restart(int nlist, const int32_t* __restrict__ ilist, int32_t* __restrict__ iter,
+                              int32_t* __restrict__ live, int32_t* __restrict__ part, const int32_t* __restrict__ total,
+                              int32_t* __restrict__ jdone, const double* __restrict__ seg, double* __restrict__ coef,
+                              int mstride, double* __restrict__ g, double* __restrict__ gs, int R, double rtol,
+                              double atol, int maxit) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int inst = ilist[t];
+  jdone[inst] = 0;
+  if (!live[inst]) { iter[inst] = 0; part[inst] = 0; return; }
+  const double beta = sqrt(seg[(int64_t)inst * mstride]);
+  double* gsi = gs + (int64_t)inst * GS_NSCAL;
+  if (gsi[GS_BETA0] < 0) gsi[GS_BETA0] = beta;
+  gsi[GS_RES] = beta;
+  const double tol = fmax(rtol * gsi[GS_BETA0], atol);
+  if (beta <= tol || total[inst] >= maxit) {
+    live[inst] = 0;
+    iter[inst] = 0;
+    part[inst] = 0;
+    return;
+  }
+  iter[inst] = 1;
+  part[inst] = 1;
+  double* gi = g + (int64_t)inst * (R + 1);
+  gi[0] = beta;
+  for (int k = 1; k <= R; ++k) gi[k] = 0.0;
+  coef[(int64_t)inst * mstride] = 1.0 / beta;
+}
+
+__global__ void gmres_init(int nlist, const int32_t* __restrict__ ilist, int32_t* __restrict__ iter,
+                           int32_t* __restrict__ live, int32_t* __restrict__ part, int32_t* __restrict__ total,
+                           double* __restrict__ gs, double* __restrict__ coef, int mstride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist) return;
+  const int inst = ilist[t];
+  iter[inst] = 0;
+  live[inst] = 1;
+  part[inst] = 0;
+  total[inst] = 0;
+  gs[(int64_t)inst * GS_NSCAL + GS_BETA0] = -1.0;
+  gs[(int64_t)inst * GS_NSCAL + GS_RES] = 0.0;
+  coef[(int64_t)inst * mstride] = 0.0;
+}
+
+// y = x, y = a - b, y += x   over the active dofs of the instances with mask 1
+__global__ void copy_masked(const double* __restrict__ x, double* __restrict__ y, const int32_t* __restrict__ el_list,
+                            const int32_t* __restrict__ el_inst, int dpe, int64_t n, const int32_t* __restrict__ mask) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t k = active_dof(el_list, t, dpe);
+    if (mask[el_inst[k / dpe]]) y[k] = x[k];
+  }
+}
+
+__global__ void sub_masked(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ y,
+                           const int32_t* __restrict__ el_list, const int32_t* __restrict__ el_inst, int dpe,
+                           int64_t n, const int32_t* __restrict__ mask) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t k = active_dof(el_list, t, dpe);
+    if (mask[el_inst[k / dpe]]) y[k] = a[k] - b[k];
+  }
+}
+
+__global__ void add_masked(const double* __restrict__ x, double* __restrict__ y, const int32_t* __restrict__ el_list,
+                           const int32_t* __restrict__ el_inst, int dpe, int64_t n, const int32_t* __restrict__ mask) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const int64_t k = active_dof(el_list, t, dpe);
+    if (mask[el_inst[k / dpe]]) y[k] += x[k];
   }
 }
 
@@ -160,6 +331,12 @@ struct pdes_batch_group {
   // solver vectors
   double *d_V = nullptr, *d_z = nullptr, *d_w = nullptr, *d_dq = nullptr, *d_rhs = nullptr;
   int restart = 0;
+  // device-resident GMRES state, per instance
+  int32_t* d_chunk_inst = nullptr;   // instance of each reduction chunk
+  int32_t *d_iter = nullptr, *d_live = nullptr, *d_part = nullptr, *d_total = nullptr, *d_jdone = nullptr;
+  double *d_H = nullptr, *d_cs = nullptr, *d_sn = nullptr, *d_g = nullptr, *d_gs = nullptr;
+  int32_t* h_flags = nullptr;        // pinned readback of d_iter / d_live / d_total
+  int gmres_R = 0;                   // restart length the H/cs/sn/g buffers are sized for
 };
 
 struct pdes_batch {
@@ -173,9 +350,11 @@ namespace {
 void group_free(pdes_batch_group* g) {
   if (!g) return;
   void* dev[] = {g->d_cbeg, g->d_clen, g->d_inst_c0, g->d_clist, g->d_ilist, g->d_partial, g->d_seg,
-                 g->d_coef, g->d_V, g->d_z, g->d_w, g->d_dq, g->d_rhs};
+                 g->d_coef, g->d_V, g->d_z, g->d_w, g->d_dq, g->d_rhs, g->d_chunk_inst, g->d_iter,
+                 g->d_live, g->d_part, g->d_total, g->d_jdone, g->d_H, g->d_cs, g->d_sn, g->d_g, g->d_gs};
   for (void* p : dev)
     if (p) hipFree(p);
+  if (g->h_flags) hipHostFree(g->h_flags);
   if (g->h_seg) hipHostFree(g->h_seg);
   if (g->h_coef) hipHostFree(g->h_coef);
   if (g->c) pdes_ctx_destroy(g->c);
@@ -218,13 +397,13 @@ int set_worklist(pdes_batch_group* g, Pick pick, std::vector<int>& chunks, std::
 
 // seg[inst][0..m) = V_i . w  (or weighted squares of V_0 when wt) for the listed instances
 int seg_dots(pdes_batch_group* g, const double* V, int m, int64_t ldv, const double* w, const double* wt,
-             int nchunks, int ninst_list) {
+             int nchunks, int ninst_list, const int32_t* mask = nullptr) {
   pdes_ctx* c = g->c;
   if (nchunks == 0) return PDES_OK;
   hipLaunchKernelGGL(seg_dot_partial, dim3(nchunks), dim3(SEG_THREADS), 0, c->stream, V, m, ldv, w, wt, g->d_cbeg,
-                     g->d_clen, g->d_clist, g->d_partial, g->mstride);
+                     g->d_clen, g->d_clist, g->d_partial, g->mstride, g->d_chunk_inst, mask);
   hipLaunchKernelGGL(seg_dot_reduce, dim3(ninst_list), dim3(64), 0, c->stream, g->d_partial, m, g->mstride,
-                     g->d_inst_c0, g->d_ilist, g->d_seg);
+                     g->d_inst_c0, g->d_ilist, g->d_seg, mask);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -256,166 +435,115 @@ int apply_op(pdes_batch_group* g, int mode, const double* x, double* y) {
   return st ? st : launch_ptc_shift(c, x, y);
 }
 
-// Batched GMRES(restart), right preconditioned, x0 = 0, for the instances
-// with active == true; b and x are packed vectors.
-int batch_gmres(pdes_batch_group* g, const pdes_newton_opts* o, const double* b, double* x) {
+// apply the operator with the preconditioner/product mask set to `mask`
+int apply_op_masked(pdes_batch_group* g, int mode, const double* x, double* y, const int32_t* mask) {
+  g->c->d_mask = mask;
+  const int st = apply_op(g, mode, x, y);
+  g->c->d_mask = nullptr;
+  return st;
+}
+
+int pc_apply_masked(pdes_batch_group* g, const double* b, double* x, const int32_t* mask) {
+  g->c->d_mask = mask;
+  const int st = pc_apply(g->c, b, x);
+  g->c->d_mask = nullptr;
+  return st;
+}
+
+// Batched GMRES(restart), right preconditioned, x0 = 0, for the instances in
+// the current work list (the Newton-active ones); b and x are packed vectors.
+// The Arnoldi steps, Givens rotations, residual estimates and convergence
+// tests of every instance run on the device: each instance carries its own
+// flags (iterating in this cycle / not yet converged / took part in this
+// cycle) that the vector kernels, the BSR product and the preconditioner
+// honour, so no host round trip happens inside a cycle except a check every
+// few steps for whether any instance is still iterating.
+int batch_gmres(pdes_batch_group* g, const pdes_newton_opts* o, const double* b, double* x,
+                const std::vector<int>& chunks, const std::vector<int>& insts) {
   pdes_ctx* c = g->c;
   const int R = o->gmres_restart;
   const int64_t ldv = c->numDof;
   const int ms = g->mstride;
-  std::vector<int> chunks, insts;
+  const int ni = (int)insts.size();
+  const int nch = (int)chunks.size();
+  if (ni == 0) return PDES_OK;
+  const int64_t n = active_dofs(g);
+  const unsigned gi = (unsigned)ceil_div(ni, 64);
+  hipStream_t sm = c->stream;
   int st;
-  for (auto& s : g->inst) {
-    s.g_iter = s.active;
-    s.g_done = !s.active;
-    s.g_total = 0;
-    s.g_beta0 = -1;
-    s.H.assign((size_t)(R + 1) * R, 0.0);
-    s.cs.assign(R, 0.0);
-    s.sn.assign(R, 0.0);
-    s.g.assign(R + 1, 0.0);
-  }
-  // x = 0 on the active instances
-  if ((st = set_worklist(g, [](const InstState& s) { return s.g_iter; }, chunks, insts))) return st;
+  hipLaunchKernelGGL(gmres_init, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_iter, g->d_live, g->d_part,
+                     g->d_total, g->d_gs, g->d_coef, ms);
+  // x = 0 (coef[inst][0] = 0 from gmres_init)
+  hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, b, x, g->d_coef, ms, 0, c->d_el_list,
+                     c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
   bool first = true;
   while (true) {
-    // r = b - A x  (x = 0 on the first cycle) into V_0, for the instances still iterating
-    if ((st = set_worklist(g, [](const InstState& s) { return !s.g_done; }, chunks, insts))) return st;
-    if (insts.empty()) break;
-    const int64_t n = active_dofs(g);
-    for (int i : insts) g->h_coef[(int64_t)i * ms] = 1.0;
-    if ((st = push_coef(g))) return st;
+    // V_0 = b - A x for the instances not yet converged
     if (first) {
-      for (int i : insts) g->h_coef[(int64_t)i * ms] = 0.0;
-      if ((st = push_coef(g))) return st;
-      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, b, x, g->d_coef, ms, 0,
-                         c->d_el_list, c->d_el_inst, g->dpe, n);  // x = 0
-      for (int i : insts) g->h_coef[(int64_t)i * ms] = 1.0;
-      if ((st = push_coef(g))) return st;
-      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, b, g->d_V, g->d_coef, ms, 0,
-                         c->d_el_list, c->d_el_inst, g->dpe, n);
+      hipLaunchKernelGGL(copy_masked, dim3(grid_n(n)), dim3(256), 0, sm, b, g->d_V, c->d_el_list, c->d_el_inst,
+                         g->dpe, n, g->d_live);
     } else {
-      if ((st = apply_op(g, o->jv_mode, x, g->d_w))) return st;
-      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, b, g->d_V, g->d_coef, ms, 0,
-                         c->d_el_list, c->d_el_inst, g->dpe, n);
-      for (int i : insts) g->h_coef[(int64_t)i * ms] = -1.0;
-      if ((st = push_coef(g))) return st;
-      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, g->d_w, g->d_V, g->d_coef, ms, 1,
-                         c->d_el_list, c->d_el_inst, g->dpe, n);
+      if ((st = apply_op_masked(g, o->jv_mode, x, g->d_w, g->d_live))) return st;
+      hipLaunchKernelGGL(sub_masked, dim3(grid_n(n)), dim3(256), 0, sm, b, g->d_w, g->d_V, c->d_el_list,
+                         c->d_el_inst, g->dpe, n, g->d_live);
     }
     first = false;
-    if ((st = seg_dots(g, g->d_V, 1, ldv, g->d_V, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
-    if ((st = fetch_seg(g))) return st;
-    for (int i : insts) {
-      InstState& s = g->inst[i];
-      const double beta = std::sqrt(g->h_seg[(int64_t)i * ms]);
-      if (s.g_beta0 < 0) s.g_beta0 = beta;
-      s.g_res = beta;
-      const double tol = std::max(o->krylov_reltol * s.g_beta0, o->krylov_abstol);
-      s.g_iter = !(beta <= tol || s.g_total >= o->krylov_itermax);
-      if (!s.g_iter) s.g_done = true;
-      std::fill(s.g.begin(), s.g.end(), 0.0);
-      s.g[0] = beta;
-      s.g_j = 0;
-      g->h_coef[(int64_t)i * ms] = s.g_iter ? 1.0 / beta : 0.0;
-    }
-    if ((st = set_worklist(g, [](const InstState& s) { return s.g_iter; }, chunks, insts))) return st;
-    if (insts.empty()) break;
-    if ((st = push_coef(g))) return st;
-    int64_t n2 = active_dofs(g);
-    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n2)), dim3(256), 0, c->stream, g->d_V, g->d_V, g->d_coef, ms, 0,
-                       c->d_el_list, c->d_el_inst, g->dpe, n2);
-    std::vector<int> cycle_insts = insts;  // instances iterating in this cycle
+    if ((st = seg_dots(g, g->d_V, 1, ldv, g->d_V, nullptr, nch, ni, g->d_live))) return st;
+    hipLaunchKernelGGL(gmres_restart, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_iter, g->d_live, g->d_part,
+                       g->d_total, g->d_jdone, g->d_seg, g->d_coef, ms, g->d_g, g->d_gs, R, o->krylov_reltol,
+                       o->krylov_abstol, o->krylov_itermax);
+    // one readback per cycle: does any instance iterate?
+    PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_iter, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
+    PDES_HIP(hipStreamSynchronize(sm));
+    bool any = false;
+    for (int i : insts) any = any || g->h_flags[i];
+    if (!any) break;
+    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, g->d_V, g->d_coef, ms, 0, c->d_el_list,
+                       c->d_el_inst, g->dpe, n, g->d_iter);
     for (int j = 0; j < R; ++j) {
-      if ((st = set_worklist(g, [](const InstState& s) { return s.g_iter; }, chunks, insts))) return st;
-      if (insts.empty()) break;
-      n2 = active_dofs(g);
       double* Vj = g->d_V + (int64_t)j * ldv;
       double* Vn = g->d_V + (int64_t)(j + 1) * ldv;
-      if ((st = pc_apply(c, Vj, g->d_z))) return st;
-      if ((st = apply_op(g, o->jv_mode, g->d_z, Vn))) return st;
-      // CGS2 with per-instance coefficients
-      std::vector<double> hsum((size_t)g->inst.size() * (j + 1), 0.0);
+      if ((st = pc_apply_masked(g, Vj, g->d_z, g->d_iter))) return st;
+      if ((st = apply_op_masked(g, o->jv_mode, g->d_z, Vn, g->d_iter))) return st;
+      // CGS2: two passes of h = V^T w, w -= V h; H column = sum of both passes
       for (int pass = 0; pass < 2; ++pass) {
-        if ((st = seg_dots(g, g->d_V, j + 1, ldv, Vn, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
-        if ((st = fetch_seg(g))) return st;
-        for (int i : insts)
-          for (int k = 0; k <= j; ++k) {
-            const double h = g->h_seg[(int64_t)i * ms + k];
-            g->h_coef[(int64_t)i * ms + k] = h;
-            hsum[(size_t)i * (j + 1) + k] += h;
-          }
-        if ((st = push_coef(g))) return st;
-        hipLaunchKernelGGL(seg_multiaxpy, dim3(grid_n(n2)), dim3(256), 0, c->stream, g->d_V, j + 1, ldv, g->d_coef, ms,
-                           Vn, c->d_el_list, c->d_el_inst, g->dpe, n2);
+        hipLaunchKernelGGL(seg_dot_partial, dim3(nch), dim3(SEG_THREADS), 0, sm, g->d_V, j + 1, ldv, Vn,
+                           (const double*)nullptr, g->d_cbeg, g->d_clen, g->d_clist, g->d_partial, ms,
+                           g->d_chunk_inst, g->d_iter);
+        hipLaunchKernelGGL(seg_reduce_hcol, dim3(ni), dim3(64), 0, sm, g->d_partial, j + 1, ms, g->d_inst_c0,
+                           g->d_ilist, g->d_iter, g->d_coef, g->d_H, R, j, pass);
+        hipLaunchKernelGGL(seg_multiaxpy, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, j + 1, ldv, g->d_coef, ms, Vn,
+                           c->d_el_list, c->d_el_inst, g->dpe, n, g->d_iter);
       }
-      if ((st = seg_dots(g, Vn, 1, ldv, Vn, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
-      if ((st = fetch_seg(g))) return st;
-      for (int i : insts) {
-        InstState& s = g->inst[i];
-        const double hn = std::sqrt(g->h_seg[(int64_t)i * ms]);
-        for (int k = 0; k <= j; ++k) s.H[(size_t)k * R + j] = hsum[(size_t)i * (j + 1) + k];
-        s.H[(size_t)(j + 1) * R + j] = hn;
-        g->h_coef[(int64_t)i * ms] = hn > 0 ? 1.0 / hn : 0.0;
-        for (int k = 0; k < j; ++k) {
-          const double a = s.H[(size_t)k * R + j], bb = s.H[(size_t)(k + 1) * R + j];
-          s.H[(size_t)k * R + j] = s.cs[k] * a + s.sn[k] * bb;
-          s.H[(size_t)(k + 1) * R + j] = -s.sn[k] * a + s.cs[k] * bb;
-        }
-        const double a = s.H[(size_t)j * R + j], bb = s.H[(size_t)(j + 1) * R + j];
-        const double r = std::hypot(a, bb);
-        s.cs[j] = r > 0 ? a / r : 1.0;
-        s.sn[j] = r > 0 ? bb / r : 0.0;
-        s.H[(size_t)j * R + j] = r;
-        s.H[(size_t)(j + 1) * R + j] = 0.0;
-        s.g[j + 1] = -s.sn[j] * s.g[j];
-        s.g[j] = s.cs[j] * s.g[j];
-        s.g_res = std::fabs(s.g[j + 1]);
-        ++s.g_total;
-        s.g_j = j + 1;
-        const double tol = std::max(o->krylov_reltol * s.g_beta0, o->krylov_abstol);
-        if (s.g_res <= tol || hn == 0.0 || s.g_total >= o->krylov_itermax) s.g_iter = false;
-        if (s.g_res <= tol || hn == 0.0) s.g_done = true;
+      if ((st = seg_dots(g, Vn, 1, ldv, Vn, nullptr, nch, ni, g->d_iter))) return st;
+      hipLaunchKernelGGL(gmres_finish_step, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_iter, g->d_live,
+                         g->d_total, g->d_jdone, g->d_seg, g->d_coef, ms, g->d_H, g->d_cs, g->d_sn, g->d_g, g->d_gs,
+                         R, j, o->krylov_reltol, o->krylov_abstol, o->krylov_itermax);
+      // instances that stopped in gmres_finish_step are masked out of the scale
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, Vn, Vn, g->d_coef, ms, 0, c->d_el_list,
+                         c->d_el_inst, g->dpe, n, g->d_iter);
+      PDES_HIP(hipGetLastError());
+      if (j + 1 < R && (j + 1) % 8 == 0) {
+        PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_iter, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
+        PDES_HIP(hipStreamSynchronize(sm));
+        bool still = false;
+        for (int i : insts) still = still || g->h_flags[i];
+        if (!still) break;
       }
-      if ((st = push_coef(g))) return st;
-      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n2)), dim3(256), 0, c->stream, Vn, Vn, g->d_coef, ms, 0,
-                         c->d_el_list, c->d_el_inst, g->dpe, n2);
     }
     // x += M^-1 V y for the instances of this cycle
-    for (int i : cycle_insts) {
-      InstState& s = g->inst[i];
-      std::vector<double> y(s.g_j);
-      for (int k = s.g_j - 1; k >= 0; --k) {
-        double v = s.g[k];
-        for (int l = k + 1; l < s.g_j; ++l) v -= s.H[(size_t)k * R + l] * y[l];
-        y[k] = v / s.H[(size_t)k * R + k];
-      }
-      for (int k = 0; k < ms; ++k) g->h_coef[(int64_t)i * ms + k] = k < s.g_j ? y[k] : 0.0;
-      if (!s.g_done && s.g_total >= o->krylov_itermax) s.g_done = true;  // iteration limit
-    }
-    {
-      std::vector<int32_t> el;
-      for (int i : cycle_insts)
-        for (int64_t e = g->inst[i].el0; e < g->inst[i].el1; ++e) el.push_back((int32_t)e);
-      c->nlist = (int32_t)el.size();
-      PDES_HIP(hipMemcpyAsync(c->d_el_list, el.data(), el.size() * 4, hipMemcpyHostToDevice, c->stream));
-      if ((st = push_coef(g))) return st;
-      const int64_t n3 = active_dofs(g);
-      hipLaunchKernelGGL(seg_combine, dim3(grid_n(n3)), dim3(256), 0, c->stream, g->d_V, R, ldv, g->d_coef, ms, g->d_w,
-                         c->d_el_list, c->d_el_inst, g->dpe, n3);
-      if ((st = pc_apply(c, g->d_w, g->d_z))) return st;
-      for (int i : cycle_insts) g->h_coef[(int64_t)i * ms] = 1.0;
-      if ((st = push_coef(g))) return st;
-      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n3)), dim3(256), 0, c->stream, g->d_z, x, g->d_coef, ms, 1,
-                         c->d_el_list, c->d_el_inst, g->dpe, n3);
-      PDES_HIP(hipStreamSynchronize(c->stream));
-    }
-    bool any = false;
-    for (int i : cycle_insts) any = any || !g->inst[i].g_done;
-    if (!any) break;
+    hipLaunchKernelGGL(gmres_solve_y, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_part, g->d_jdone, g->d_H,
+                       g->d_g, g->d_coef, ms, R);
+    hipLaunchKernelGGL(seg_combine, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, R, ldv, g->d_coef, ms, g->d_w,
+                       c->d_el_list, c->d_el_inst, g->dpe, n, g->d_part);
+    if ((st = pc_apply_masked(g, g->d_w, g->d_z, g->d_part))) return st;
+    hipLaunchKernelGGL(add_masked, dim3(grid_n(n)), dim3(256), 0, sm, g->d_z, x, c->d_el_list, c->d_el_inst, g->dpe,
+                       n, g->d_part);
+    PDES_HIP(hipGetLastError());
   }
-  for (auto& s : g->inst)
-    if (s.active) s.rep.total_krylov_iters += s.g_total;
+  PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_total, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
+  PDES_HIP(hipStreamSynchronize(sm));
+  for (int i : insts) g->inst[i].rep.total_krylov_iters += g->h_flags[i];
   return PDES_OK;
 }
 
@@ -480,14 +608,14 @@ int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
     for (int i : insts) g->h_coef[(int64_t)i * g->mstride] = -1.0;
     if ((st = push_coef(g))) return st;
     hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, c->d_res, g->d_rhs, g->d_coef, g->mstride,
-                       0, c->d_el_list, c->d_el_inst, g->dpe, n);
-    if ((st = batch_gmres(g, o, g->d_rhs, g->d_dq))) return st;
-    // q += dq, step norms, new residual norms for the active instances
-    if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
+                       0, c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
+    if ((st = batch_gmres(g, o, g->d_rhs, g->d_dq, chunks, insts))) return st;
+    // q += dq, step norms, new residual norms for the active instances (the
+    // work list is unchanged by the linear solve)
     for (int i : insts) g->h_coef[(int64_t)i * g->mstride] = 1.0;
     if ((st = push_coef(g))) return st;
     hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, g->d_dq, c->d_q, g->d_coef, g->mstride, 1,
-                       c->d_el_list, c->d_el_inst, g->dpe, n);
+                       c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
     if ((st = seg_dots(g, g->d_dq, 1, 0, g->d_dq, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
     if ((st = fetch_seg(g))) return st;
     for (int i : insts) g->inst[i].rep.step_norm = std::sqrt(g->h_seg[(int64_t)i * g->mstride]);
@@ -646,7 +774,23 @@ int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_de
     dm((void**)&g->d_w, c->numDof * 8);
     dm((void**)&g->d_dq, c->numDof * 8);
     dm((void**)&g->d_rhs, c->numDof * 8);
+    dm((void**)&g->d_chunk_inst, nchunk * 4);
+    for (int32_t** f : {&g->d_iter, &g->d_live, &g->d_part, &g->d_total, &g->d_jdone}) dm((void**)f, ni * 4);
+    g->gmres_R = g->mstride - 2;
+    dm((void**)&g->d_H, (size_t)ni * (g->gmres_R + 1) * g->gmres_R * 8);
+    dm((void**)&g->d_cs, (size_t)ni * g->gmres_R * 8);
+    dm((void**)&g->d_sn, (size_t)ni * g->gmres_R * 8);
+    dm((void**)&g->d_g, (size_t)ni * (g->gmres_R + 1) * 8);
+    dm((void**)&g->d_gs, (size_t)ni * GS_NSCAL * 8);
     if (st) break;
+    if (hipHostMalloc((void**)&g->h_flags, (size_t)ni * 4) != hipSuccess) {
+      set_error("pinned allocation failed");
+      st = PDES_ERR_NOMEM;
+      break;
+    }
+    std::vector<int32_t> chunk_inst(nchunk);
+    for (int i = 0; i < ni; ++i)
+      for (int ch = g->h_inst_c0[i]; ch < g->h_inst_c0[i + 1]; ++ch) chunk_inst[ch] = i;
     if (hipHostMalloc((void**)&g->h_seg, (size_t)ni * g->mstride * 8) != hipSuccess ||
         hipHostMalloc((void**)&g->h_coef, (size_t)ni * g->mstride * 8) != hipSuccess) {
       set_error("pinned allocation failed");
@@ -661,6 +805,7 @@ int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_de
         hipMemcpyAsync(g->d_cbeg, g->h_cbeg.data(), nchunk * 8, hipMemcpyHostToDevice, s) ||
         hipMemcpyAsync(g->d_clen, g->h_clen.data(), nchunk * 4, hipMemcpyHostToDevice, s) ||
         hipMemcpyAsync(g->d_inst_c0, g->h_inst_c0.data(), (ni + 1) * 4, hipMemcpyHostToDevice, s) ||
+        hipMemcpyAsync(g->d_chunk_inst, chunk_inst.data(), nchunk * 4, hipMemcpyHostToDevice, s) ||
         hipMemsetAsync(c->d_flag, 0, ni * 4, s) || hipStreamSynchronize(s)) {
       set_error("batch upload failed");
       st = PDES_ERR_HIP;

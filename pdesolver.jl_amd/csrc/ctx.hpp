@@ -53,6 +53,9 @@ struct pdes_ctx {
   double* d_ptc_diag = nullptr;
   double* d_ptc_scale = nullptr;
   bool ptc_on = false;
+  // packed batches: per-instance 0/1 mask honoured by the BSR product and the
+  // preconditioner apply (instances with 0 are skipped); null = all
+  const int32_t* d_mask = nullptr;
   int partial_n = 0;
   // residual launch variants (see residual_launch.hpp) for the plain and the
   // RK-epilogue kernel, chosen by the autotuner or the default table

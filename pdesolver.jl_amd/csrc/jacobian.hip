@@ -107,11 +107,11 @@ static int launch_spmv_nd(pdes_ctx* c, const double* x, double* y, double alpha,
   if (j->fp64)
     hipLaunchKernelGGL((bsr_spmv_kernel<ND, double>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, nrows,
                        j->d_rowptr, j->d_colind, (const double*)j->d_vals, x, y, alpha, yadd, beta,
-                       c->active_list(), c->Np);
+                       c->active_list(), c->Np, c->d_el_inst, c->d_mask);
   else
     hipLaunchKernelGGL((bsr_spmv_kernel<ND, float>), dim3((unsigned)blocks), dim3(threads), 0, c->stream, nrows,
                        j->d_rowptr, j->d_colind, (const float*)j->d_vals, x, y, alpha, yadd, beta,
-                       c->active_list(), c->Np);
+                       c->active_list(), c->Np, c->d_el_inst, c->d_mask);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }

@@ -325,13 +325,16 @@ __global__ void __launch_bounds__(256)
 bsr_spmv_kernel(int64_t nrows, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
                 const BT* __restrict__ vals, const double* __restrict__ x, double* __restrict__ y,
                 double alpha, const double* __restrict__ yadd, double beta,
-                const int32_t* __restrict__ el_list, int np) {
+                const int32_t* __restrict__ el_list, int np,
+                const int32_t* __restrict__ el_inst, const int32_t* __restrict__ mask) {
   const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t rr = tt / ND;
   const int a = (int)(tt - rr * ND);
   if (rr >= nrows) return;
   // with an element list, launch row rr is node (rr % np) of element el_list[rr / np]
   const int64_t row = el_list ? (int64_t)el_list[rr / np] * np + (rr % np) : rr;
+  // packed batches: rows of instances whose mask entry is 0 are left untouched
+  if (mask && !mask[el_inst[row / np]]) return;
   const int64_t t = row * ND + a;
   double acc = 0.0;
   const int64_t b0 = rowptr[row], b1 = rowptr[row + 1];

@@ -1,8 +1,10 @@
 // solver.hip -- preconditioners, GMRES and Newton on the device (C ABI).
 //
 // NewtonBDiagPC (src/NonlinearSolvers/preconditioning_types.jl:41-110,
-// preconditioning.jl:7-215): LU of each element's diagonal (Np*ndof)^2 block of
-// the Jacobian, applied element by element.  Factor and apply run in FP32.
+// preconditioning.jl:7-215): each element's diagonal (Np*ndof)^2 block of the
+// Jacobian is inverted (the reference factors it with getrf and applies getrs;
+// an explicit inverse makes the apply one coalesced product per element).
+// Setup and apply run in FP32.
 // GMRES(restart) with right preconditioning replaces the PETSc KSP of
 // src/linearsolvers/ls_standard.jl:321-370 (KSP options read_input.jl:560-581);
 // the Arnoldi basis, Hessenberg and update stay FP64.  Orthogonalisation is
@@ -17,8 +19,7 @@
 struct pdes_precond {
   int kind = 0;
   int n = 0;                 // block size Np*nd (block-Jacobi)
-  float* d_lu = nullptr;     // [numEl][n][n] column-major LU
-  int32_t* d_piv = nullptr;  // [numEl][n]
+  float* d_inv = nullptr;    // [numEl][n][n] column-major inverse of each element block
   bool factored = false;
 };
 
@@ -30,26 +31,93 @@ void gmres_release(pdes_ctx* c);
 int precond_free(pdes_ctx* c) {
   gmres_release(c);
   if (!c->pc_obj) return PDES_OK;
-  if (c->pc_obj->d_lu) hipFree(c->pc_obj->d_lu);
-  if (c->pc_obj->d_piv) hipFree(c->pc_obj->d_piv);
+  if (c->pc_obj->d_inv) hipFree(c->pc_obj->d_inv);
   delete c->pc_obj;
   c->pc_obj = nullptr;
   return PDES_OK;
 }
 
 // ---------------------------------------------------------------------------
-// block-Jacobi factor: gather the element block from the BSR matrix into LDS
-// (FP32), LU with partial pivoting, store column-major.
+// block-Jacobi setup: gather the element block from the BSR matrix into LDS
+// (FP32) and invert it in place by Gauss-Jordan elimination with partial
+// (row) pivoting; the inverse is stored column-major so that the apply is a
+// coalesced matrix-vector product.
 // ---------------------------------------------------------------------------
+template <int N>
+__device__ void gauss_jordan_inverse(float (*A)[N + 1], int* perm, float* fcol, float* redv, int* redi, int* pivrow,
+                                     int32_t* err) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < N; ++k) {
+    // pivot: max |A[r][k]| over r >= k, lowest row on ties (first wave)
+    if (tid < 64) {
+      float best = -1.0f;
+      int bi = k;
+      for (int r = k + tid; r < N; r += 64) {
+        const float v = fabsf(A[r][k]);
+        if (v > best) { best = v; bi = r; }
+      }
+      redv[tid] = best;
+      redi[tid] = bi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      float best = -1.0f;
+      int bi = k;
+      for (int t = 0; t < 64; ++t)
+        if (redv[t] > best || (redv[t] == best && redi[t] < bi)) { best = redv[t]; bi = redi[t]; }
+      *pivrow = bi;
+      perm[k] = bi;
+      if (!(best > 0.0f)) err[0] = 1;
+    }
+    __syncthreads();
+    const int p = *pivrow;
+    if (p != k)
+      for (int cc = tid; cc < N; cc += blockDim.x) {
+        const float t0 = A[k][cc];
+        A[k][cc] = A[p][cc];
+        A[p][cc] = t0;
+      }
+    __syncthreads();
+    const float pinv = 1.0f / A[k][k];
+    __syncthreads();
+    // column k of the identity takes the pivot's place, then row k is scaled
+    if (tid == 0) A[k][k] = 1.0f;
+    for (int r = tid; r < N; r += blockDim.x) fcol[r] = (r == k) ? 0.0f : A[r][k];
+    __syncthreads();
+    for (int cc = tid; cc < N; cc += blockDim.x) A[k][cc] *= pinv;
+    for (int r = tid; r < N; r += blockDim.x)
+      if (r != k) A[r][k] = 0.0f;
+    __syncthreads();
+    for (int t = tid; t < N * N; t += blockDim.x) {
+      const int r = t / N, cc = t - r * N;
+      if (r != k) A[r][cc] -= fcol[r] * A[k][cc];
+    }
+    __syncthreads();
+  }
+  // undo the row interchanges as column interchanges, last first
+  for (int k = N - 1; k >= 0; --k) {
+    const int p = perm[k];
+    if (p != k)
+      for (int r = tid; r < N; r += blockDim.x) {
+        const float t0 = A[r][k];
+        A[r][k] = A[r][p];
+        A[r][p] = t0;
+      }
+    __syncthreads();
+  }
+}
+
 template <int NP, int ND, class BT>
 __global__ void __launch_bounds__(256)
 bjacobi_factor_kernel(const int32_t* __restrict__ el_list, const int64_t* __restrict__ rowptr,
-                      const int32_t* __restrict__ colind, const BT* __restrict__ vals, float* __restrict__ lu,
-                      int32_t* __restrict__ piv, int32_t* __restrict__ err) {
+                      const int32_t* __restrict__ colind, const BT* __restrict__ vals, float* __restrict__ inv,
+                      int32_t* __restrict__ err) {
   constexpr int N = NP * ND;
   __shared__ float A[N][N + 1];
+  __shared__ float fcol[N];
   __shared__ float redv[64];
   __shared__ int redi[64];
+  __shared__ int perm[N];
   __shared__ int pivrow;
   const int tid = threadIdx.x;
   const int64_t e = el_list ? (int64_t)el_list[blockIdx.x] : (int64_t)blockIdx.x;
@@ -62,85 +130,42 @@ bjacobi_factor_kernel(const int32_t* __restrict__ el_list, const int64_t* __rest
         A[i * ND + a][j * ND + b] = pos >= 0 ? (float)vals[pos * ND * ND + a * ND + b] : 0.0f;
   }
   __syncthreads();
-  for (int k = 0; k < N; ++k) {
-    // pivot: max |A[r][k]|, r >= k (first wave)
-    if (tid < 64) {
-      float best = -1.0f;
-      int bi = k;
-      for (int r = k + tid; r < N; r += 64) {
-        float v = fabsf(A[r][k]);
-        if (v > best) { best = v; bi = r; }
-      }
-      redv[tid] = best;
-      redi[tid] = bi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      float best = -1.0f;
-      int bi = k;
-      for (int t = 0; t < 64; ++t)
-        if (redv[t] > best || (redv[t] == best && redi[t] < bi)) { best = redv[t]; bi = redi[t]; }
-      pivrow = bi;
-      piv[e * N + k] = bi;
-      if (!(best > 0.0f)) err[0] = 1;
-    }
-    __syncthreads();
-    const int p = pivrow;
-    if (p != k)
-      for (int cc = tid; cc < N; cc += blockDim.x) {
-        float t0 = A[k][cc];
-        A[k][cc] = A[p][cc];
-        A[p][cc] = t0;
-      }
-    __syncthreads();
-    const float dinv = 1.0f / A[k][k];
-    for (int r = k + 1 + tid; r < N; r += blockDim.x) A[r][k] *= dinv;
-    __syncthreads();
-    const int m = N - k - 1;
-    for (int t = tid; t < m * m; t += blockDim.x) {
-      const int r = k + 1 + t / m, cc = k + 1 + t % m;
-      A[r][cc] -= A[r][k] * A[k][cc];
-    }
-    __syncthreads();
-  }
-  float* out = lu + e * N * N;
+  gauss_jordan_inverse<N>(A, perm, fcol, redv, redi, &pivrow, err);
+  float* out = inv + e * N * N;
   for (int t = tid; t < N * N; t += blockDim.x) {
     const int cc = t / N, r = t - cc * N;
     out[t] = A[r][cc];
   }
 }
 
-// block-Jacobi apply: one wave per element, lane r owns rows r, r+64, ...
+// block-Jacobi apply: x_e = inv_e b_e; EPB = 256 / N elements per block, thread
+// (element slot, row r) forms one row of the product from the column-major
+// inverse (consecutive rows read consecutive floats).
 template <int NP, int ND>
-__global__ void __launch_bounds__(64)
-bjacobi_apply_kernel(const int32_t* __restrict__ el_list, const float* __restrict__ lu,
-                     const int32_t* __restrict__ piv, const double* __restrict__ b, double* __restrict__ x) {
+__global__ void __launch_bounds__(256)
+bjacobi_apply_kernel(const int32_t* __restrict__ el_list, int64_t nel, const float* __restrict__ inv,
+                     const double* __restrict__ b, double* __restrict__ x, const int32_t* __restrict__ el_inst,
+                     const int32_t* __restrict__ mask) {
   constexpr int N = NP * ND;
-  __shared__ float v[N];
-  const int lane = threadIdx.x;
-  const int64_t e = el_list ? (int64_t)el_list[blockIdx.x] : (int64_t)blockIdx.x;
-  for (int r = lane; r < N; r += 64) v[r] = (float)b[e * N + r];
-  __syncthreads();
-  if (lane == 0)
-    for (int k = 0; k < N; ++k) {
-      const int p = piv[e * N + k];
-      if (p != k) { float t = v[k]; v[k] = v[p]; v[p] = t; }
-    }
-  __syncthreads();
-  const float* L = lu + e * N * N;
-  for (int k = 0; k < N; ++k) {
-    const float xk = v[k];
-    for (int r = k + 1 + lane; r < N; r += 64) v[r] -= L[k * N + r] * xk;
-    __syncthreads();
+  constexpr int EPB = 256 / N;
+  __shared__ float v[EPB][N];
+  const int slot = threadIdx.x / N;
+  const int r = threadIdx.x - slot * N;
+  const int64_t s = (int64_t)blockIdx.x * EPB + slot;
+  bool act = slot < EPB && s < nel;
+  int64_t e = 0;
+  if (act) {
+    e = el_list ? (int64_t)el_list[s] : s;
+    if (mask && !mask[el_inst[e]]) act = false;
   }
-  for (int k = N - 1; k >= 0; --k) {
-    if (lane == 0) v[k] = v[k] / L[k * N + k];
-    __syncthreads();
-    const float xk = v[k];
-    for (int r = lane; r < k; r += 64) v[r] -= L[k * N + r] * xk;
-    __syncthreads();
-  }
-  for (int r = lane; r < N; r += 64) x[e * N + r] = (double)v[r];
+  if (act) v[slot][r] = (float)b[e * N + r];
+  __syncthreads();
+  if (!act) return;
+  const float* M = inv + e * N * N;
+  float acc = 0.0f;
+#pragma unroll 4
+  for (int cc = 0; cc < N; ++cc) acc += M[cc * N + r] * v[slot][cc];
+  x[e * N + r] = (double)acc;
 }
 
 template <int NP, int ND>
@@ -151,10 +176,10 @@ static int bj_factor(pdes_ctx* c) {
   const dim3 grid((unsigned)c->active_count());
   if (j->fp64)
     hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, double>), grid, dim3(256), 0, c->stream, c->active_list(),
-                       j->d_rowptr, j->d_colind, (const double*)j->d_vals, p->d_lu, p->d_piv, j->d_err);
+                       j->d_rowptr, j->d_colind, (const double*)j->d_vals, p->d_inv, j->d_err);
   else
     hipLaunchKernelGGL((bjacobi_factor_kernel<NP, ND, float>), grid, dim3(256), 0, c->stream, c->active_list(),
-                       j->d_rowptr, j->d_colind, (const float*)j->d_vals, p->d_lu, p->d_piv, j->d_err);
+                       j->d_rowptr, j->d_colind, (const float*)j->d_vals, p->d_inv, j->d_err);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -162,8 +187,10 @@ static int bj_factor(pdes_ctx* c) {
 template <int NP, int ND>
 static int bj_apply(pdes_ctx* c, const double* b, double* x) {
   if (c->active_count() == 0) return PDES_OK;
-  hipLaunchKernelGGL((bjacobi_apply_kernel<NP, ND>), dim3((unsigned)c->active_count()), dim3(64), 0, c->stream,
-                     c->active_list(), c->pc_obj->d_lu, c->pc_obj->d_piv, b, x);
+  constexpr int EPB = 256 / (NP * ND);
+  const int64_t nel = c->active_count();
+  hipLaunchKernelGGL((bjacobi_apply_kernel<NP, ND>), dim3((unsigned)ceil_div(nel, EPB)), dim3(256), 0, c->stream,
+                     c->active_list(), nel, c->pc_obj->d_inv, b, x, c->d_el_inst, c->d_mask);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -233,12 +260,10 @@ int pc_setup(pdes_ctx* c, int kind) {
   if (kind == PDES_PC_NONE) { p->factored = true; return PDES_OK; }
   PDES_CHECK(c->jac && c->jac->assembled, PDES_ERR_ARG, "assemble the Jacobian before the preconditioner");
   const int n = c->Np * c->nd;
-  if (!p->d_lu || p->n != n) {
-    if (p->d_lu) hipFree(p->d_lu);
-    if (p->d_piv) hipFree(p->d_piv);
+  if (!p->d_inv || p->n != n) {
+    if (p->d_inv) hipFree(p->d_inv);
     p->n = n;
-    if (hipMalloc((void**)&p->d_lu, (size_t)c->numEl * n * n * sizeof(float)) != hipSuccess ||
-        hipMalloc((void**)&p->d_piv, (size_t)c->numEl * n * sizeof(int32_t)) != hipSuccess) {
+    if (hipMalloc((void**)&p->d_inv, (size_t)c->numEl * n * n * sizeof(float)) != hipSuccess) {
       set_error("device allocation of the block-Jacobi factors failed");
       return PDES_ERR_NOMEM;
     }
