@@ -6,7 +6,8 @@ Workload: 96 instances (seed 3), grid edge count uniform in 8..128, Omega p in
 {1,2,3} with the Roe weak form or diagonal-E p in {2,3} with the IR split form
 and IRSLF faces, inflow Mach uniform in [0.3, 0.95], exact state times
 (1 + U(-1e-3, 1e-3)).  Each solve is Newton to ||R||_{M^-1} <= 1e-10 with FP32
-Jacobian blocks and FP32 block-Jacobi under FP64 GMRES.
+Jacobian blocks and an FP32 preconditioner (element-block ILU(0) by default,
+or element block-Jacobi) under FP64 GMRES(30).
 
 Both paths build their device objects before the timed region; the timed
 region is state upload + Newton solve (+ nothing else) for every instance.
@@ -27,20 +28,23 @@ if HERE not in sys.path:
 
 import numpy as np  # noqa: E402
 
-from pdes_amd import euler, linsolve, problems  # noqa: E402
+from pdes_amd import abi, euler, linsolve, problems  # noqa: E402
 from pdes_amd.batch import BatchSolver  # noqa: E402
 
 
-def newton_options():
-    return linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=50)
+PCS = {"bjacobi": abi.PDES_PC_BJACOBI, "ilu0": abi.PDES_PC_ILU0}
+
+
+def newton_options(pc: str = "ilu0"):
+    return linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=50, pc_kind=PCS[pc])
 
 
 def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_tol: float = 1e-8,
-        only: str = "both"):
+        only: str = "both", pc: str = "ilu0"):
     t0 = time.perf_counter()
     inst = problems.config3_instances(count=count, seed=seed, nmin=nmin, nmax=nmax)
     t_gen = time.perf_counter() - t0
-    opts = newton_options()
+    opts = newton_options(pc)
     ndof = [int(q.size) for (m, op, ph, q) in inst]
 
     if only == "batched":
@@ -91,6 +95,7 @@ def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_to
     return {
         "workload": f"{count} 2D vortex instances (seed {seed}), n in {nmin}..{nmax}, Omega p1-3 Roe / "
                     "diagonal-E p2-3 IR split, Min in [0.3,0.95], Newton to ||R|| <= 1e-10",
+        "preconditioner": pc,
         "instances": count,
         "total_dofs": int(sum(ndof)),
         "groups": groups,
@@ -120,8 +125,9 @@ def main():
     ap.add_argument("--nmin", type=int, default=8)
     ap.add_argument("--nmax", type=int, default=128)
     ap.add_argument("--only", choices=["both", "batched"], default="both")
+    ap.add_argument("--pc", choices=sorted(PCS), default="ilu0")
     a = ap.parse_args()
-    print(json.dumps(run(a.count, a.seed, a.nmin, a.nmax, only=a.only)), flush=True)
+    print(json.dumps(run(a.count, a.seed, a.nmin, a.nmax, only=a.only, pc=a.pc)), flush=True)
 
 
 if __name__ == "__main__":

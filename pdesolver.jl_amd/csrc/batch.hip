@@ -336,6 +336,7 @@ struct pdes_batch_group {
   int32_t *d_iter = nullptr, *d_live = nullptr, *d_part = nullptr, *d_total = nullptr, *d_jdone = nullptr;
   double *d_H = nullptr, *d_cs = nullptr, *d_sn = nullptr, *d_g = nullptr, *d_gs = nullptr;
   int32_t* h_flags = nullptr;        // pinned readback of d_iter / d_live / d_total
+  std::vector<int32_t> h_active;     // host copy of the ctx's per-instance work-list flags
   int gmres_R = 0;                   // restart length the H/cs/sn/g buffers are sized for
 };
 
@@ -385,6 +386,9 @@ int set_worklist(pdes_batch_group* g, Pick pick, std::vector<int>& chunks, std::
   }
   pdes_ctx* c = g->c;
   c->nlist = (int32_t)el.size();
+  for (size_t i = 0; i < g->inst.size(); ++i) g->h_active[i] = 0;
+  for (int i : insts) g->h_active[i] = 1;
+  PDES_HIP(hipMemcpyAsync(c->d_inst_active, g->h_active.data(), g->inst.size() * 4, hipMemcpyHostToDevice, c->stream));
   if (!el.empty()) PDES_HIP(hipMemcpyAsync(c->d_el_list, el.data(), el.size() * 4, hipMemcpyHostToDevice, c->stream));
   if (!chunks.empty())
     PDES_HIP(hipMemcpyAsync(g->d_clist, chunks.data(), chunks.size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -759,6 +763,8 @@ int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_de
     dm((void**)&c->d_el_list, nel * 4);
     dm((void**)&c->d_el_inst, nel * 4);
     dm((void**)&c->d_inst_phys, ni * sizeof(PhysConst));
+    dm((void**)&c->d_inst_active, ni * 4);
+    g->h_active.assign(ni, 1);
     hipFree(c->d_ptc_scale);
     c->d_ptc_scale = nullptr;
     dm((void**)&c->d_ptc_scale, ni * 8);

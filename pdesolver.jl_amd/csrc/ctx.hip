@@ -228,7 +228,8 @@ int pdes_ctx_destroy(pdes_ctx* c) {
   void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q,
                   c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_q,
                   c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
-                  c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale};
+                  c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale,
+                  c->d_inst_active};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);

@@ -56,6 +56,9 @@ struct pdes_ctx {
   // packed batches: per-instance 0/1 mask honoured by the BSR product and the
   // preconditioner apply (instances with 0 are skipped); null = all
   const int32_t* d_mask = nullptr;
+  // packed batches: 0/1 per instance, 1 while the instance is in the Newton
+  // work list (kernels that walk fixed element lists skip the others)
+  int32_t* d_inst_active = nullptr;
   int partial_n = 0;
   // residual launch variants (see residual_launch.hpp) for the plain and the
   // RK-epilogue kernel, chosen by the autotuner or the default table

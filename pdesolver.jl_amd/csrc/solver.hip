@@ -11,16 +11,24 @@
 // classical Gram-Schmidt applied twice (CGS2): two multi-dot and two
 // multi-axpy passes per step, each reading the basis once.
 // Newton follows newtonInner (src/NonlinearSolvers/newton.jl:137-304).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <vector>
 
 #include "jacobian.hpp"
 
 struct pdes_precond {
   int kind = 0;
-  int n = 0;                 // block size Np*nd (block-Jacobi)
-  float* d_inv = nullptr;    // [numEl][n][n] column-major inverse of each element block
+  int n = 0;                 // block size Np*nd
+  float* d_inv = nullptr;    // [numEl][n][n] column-major inverse of each (modified) element block
   bool factored = false;
+  // ILU(0): elements grouped by dependency level of the forward (lower
+  // neighbours first) and backward (upper neighbours first) sweeps
+  bool ilu_ready = false;
+  std::vector<int64_t> fptr, bptr;
+  int32_t* d_fel = nullptr;
+  int32_t* d_bel = nullptr;
 };
 
 namespace pdes {
@@ -32,6 +40,8 @@ int precond_free(pdes_ctx* c) {
   gmres_release(c);
   if (!c->pc_obj) return PDES_OK;
   if (c->pc_obj->d_inv) hipFree(c->pc_obj->d_inv);
+  if (c->pc_obj->d_fel) hipFree(c->pc_obj->d_fel);
+  if (c->pc_obj->d_bel) hipFree(c->pc_obj->d_bel);
   delete c->pc_obj;
   c->pc_obj = nullptr;
   return PDES_OK;
@@ -168,6 +178,244 @@ bjacobi_apply_kernel(const int32_t* __restrict__ el_list, int64_t nel, const flo
   x[e * N + r] = (double)acc;
 }
 
+// ---------------------------------------------------------------------------
+// Element-block ILU(0) in the natural element order.  With E the element
+// blocks of the Jacobian, L/U its strictly lower/upper element blocks and
+//   D'_e = A_ee - sum_{k in lower(e)} A_ek D'_k^-1 A_ke,
+// the preconditioner is M = (D' + L) D'^-1 (D' + U); no fill outside the
+// element-block pattern is kept (ILU(0) on element blocks).  A_ek couples only
+// the face-stencil nodes of e and of its neighbour k, so the update of D'_e
+// touches only the (stencil x stencil) part of the block:
+//   C = S_ek [D'_k^-1]_(Rk,Rk) S_ke,  Rk / Ce = stencil nodes of the face.
+// Elements are processed level by level (a level holds elements whose lower
+// neighbours are all in earlier levels), each level in one launch.
+// ---------------------------------------------------------------------------
+template <int NP, int ND, int NS, class BT>
+__global__ void __launch_bounds__(256)
+ilu_factor_level_kernel(const int32_t* __restrict__ lev_el, const int64_t* __restrict__ rowptr,
+                        const int32_t* __restrict__ colind, const BT* __restrict__ vals,
+                        const pdes::ElFace* __restrict__ elface, int nfaces, const int32_t* __restrict__ perm,
+                        float* __restrict__ inv, int32_t* __restrict__ err, const int32_t* __restrict__ el_inst,
+                        const int32_t* __restrict__ inst_active) {
+  constexpr int N = NP * ND;
+  constexpr int M = NS * ND;
+  __shared__ float A[N][N + 1];
+  __shared__ float B1[M][M + 1];
+  __shared__ float B2[M][M + 1];
+  __shared__ float B3[M][M + 1];
+  __shared__ float fcol[N];
+  __shared__ float redv[64];
+  __shared__ int redi[64];
+  __shared__ int permk[N];
+  __shared__ int pivrow;
+  __shared__ int Ce[NS], Rk[NS];
+  const int tid = threadIdx.x;
+  const int64_t e = lev_el[blockIdx.x];
+  if (inst_active && !inst_active[el_inst[e]]) return;
+  for (int t = tid; t < NP * NP; t += blockDim.x) {
+    const int i = t / NP, j = t - i * NP;
+    const int64_t pos = bsr_find(rowptr, colind, e * NP + i, (int32_t)(e * NP + j));
+    for (int a = 0; a < ND; ++a)
+      for (int b = 0; b < ND; ++b)
+        A[i * ND + a][j * ND + b] = pos >= 0 ? (float)vals[pos * ND * ND + a * ND + b] : 0.0f;
+  }
+  __syncthreads();
+  for (int f = 0; f < nfaces; ++f) {
+    const pdes::ElFace r = elface[e * nfaces + f];
+    if (r.x < 0 || r.x >= e) continue;   // boundary face or upper neighbour (uniform over the block)
+    const int64_t k = r.x;
+    const int fk = r.z & 0xff;
+    if (tid < NS) {
+      Ce[tid] = perm[tid * nfaces + f];
+      Rk[tid] = perm[tid * nfaces + fk];
+    }
+    __syncthreads();
+    // B1 = [D'_k^-1](Rk,Rk), B2 = S_ke = A(k rows Rk, e cols Ce)
+    const float* Ik = inv + k * N * N;
+    for (int t = tid; t < M * M; t += blockDim.x) {
+      const int r1 = t / M, c1 = t - r1 * M;
+      const int s1 = r1 / ND, a = r1 - s1 * ND, s2 = c1 / ND, b = c1 - s2 * ND;
+      B1[r1][c1] = Ik[(Rk[s2] * ND + b) * N + Rk[s1] * ND + a];
+      const int64_t pos = bsr_find(rowptr, colind, k * NP + Rk[s1], (int32_t)(e * NP + Ce[s2]));
+      B2[r1][c1] = pos >= 0 ? (float)vals[pos * ND * ND + a * ND + b] : 0.0f;
+    }
+    __syncthreads();
+    for (int t = tid; t < M * M; t += blockDim.x) {
+      const int r1 = t / M, c1 = t - r1 * M;
+      float acc = 0.0f;
+      for (int l = 0; l < M; ++l) acc += B1[r1][l] * B2[l][c1];
+      B3[r1][c1] = acc;
+    }
+    __syncthreads();
+    // B1 = S_ek = A(e rows Ce, k cols Rk)
+    for (int t = tid; t < M * M; t += blockDim.x) {
+      const int r1 = t / M, c1 = t - r1 * M;
+      const int s1 = r1 / ND, a = r1 - s1 * ND, s2 = c1 / ND, b = c1 - s2 * ND;
+      const int64_t pos = bsr_find(rowptr, colind, e * NP + Ce[s1], (int32_t)(k * NP + Rk[s2]));
+      B1[r1][c1] = pos >= 0 ? (float)vals[pos * ND * ND + a * ND + b] : 0.0f;
+    }
+    __syncthreads();
+    for (int t = tid; t < M * M; t += blockDim.x) {
+      const int r1 = t / M, c1 = t - r1 * M;
+      float acc = 0.0f;
+      for (int l = 0; l < M; ++l) acc += B1[r1][l] * B3[l][c1];
+      const int s1 = r1 / ND, a = r1 - s1 * ND, s2 = c1 / ND, b = c1 - s2 * ND;
+      A[Ce[s1] * ND + a][Ce[s2] * ND + b] -= acc;
+    }
+    __syncthreads();
+  }
+  gauss_jordan_inverse<N>(A, permk, fcol, redv, redi, &pivrow, err);
+  float* out = inv + e * N * N;
+  for (int t = tid; t < N * N; t += blockDim.x) {
+    const int cc = t / N, r = t - cc * N;
+    out[t] = A[r][cc];
+  }
+}
+
+// One level of a sweep.  Forward: x_e = D'_e^-1 (b_e - sum_{k<e} A_ek x_k).
+// Backward (in place on x): x_e -= D'_e^-1 sum_{k>e} A_ek x_k.  The neighbour
+// products walk the element's BSR rows and keep the blocks whose column
+// element is below (forward) or above (backward) e.
+template <int NP, int ND, class BT>
+__global__ void __launch_bounds__(256)
+ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const int64_t* __restrict__ rowptr,
+                       const int32_t* __restrict__ colind, const BT* __restrict__ vals, const float* __restrict__ inv,
+                       const double* __restrict__ b, double* __restrict__ x, int forward,
+                       const int32_t* __restrict__ el_inst, const int32_t* __restrict__ mask,
+                       const int32_t* __restrict__ inst_active) {
+  constexpr int N = NP * ND;
+  constexpr int EPB = 256 / N;
+  __shared__ float v[EPB][N];
+  const int slot = threadIdx.x / N;
+  const int r = threadIdx.x - slot * N;
+  const int64_t s = (int64_t)blockIdx.x * EPB + slot;
+  bool act = slot < EPB && s < count;
+  int64_t e = 0;
+  if (act) {
+    e = lev_el[s];
+    const int inst = el_inst ? el_inst[e] : 0;
+    if ((mask && !mask[inst]) || (inst_active && !inst_active[inst])) act = false;
+  }
+  if (act) {
+    const int i = r / ND, a = r - i * ND;
+    const int64_t row = e * NP + i;
+    double acc = forward ? b[e * N + r] : 0.0;
+    for (int64_t p = rowptr[row]; p < rowptr[row + 1]; ++p) {
+      const int64_t col = colind[p];
+      const int64_t ce = col / NP;
+      if (forward ? (ce >= e) : (ce <= e)) continue;
+      const BT* blk = vals + p * ND * ND + a * ND;
+      const double* xc = x + col * ND;
+      double t = 0.0;
+#pragma unroll
+      for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
+      acc = forward ? acc - t : acc + t;
+    }
+    v[slot][r] = (float)acc;
+  }
+  __syncthreads();
+  if (!act) return;
+  const float* Mi = inv + e * N * N;
+  float y = 0.0f;
+#pragma unroll 4
+  for (int cc = 0; cc < N; ++cc) y += Mi[cc * N + r] * v[slot][cc];
+  if (forward) x[e * N + r] = (double)y;
+  else x[e * N + r] -= (double)y;
+}
+
+// dependency levels of the natural-order sweeps (host, once per context)
+static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
+  const int nel = c->numEl, nf = c->nfaces;
+  std::vector<int32_t> fl(nel, 0), bl(nel, 0);
+  int fmax = 0, bmax = 0;
+  for (int e = 0; e < nel; ++e) {
+    int lv = 0;
+    for (int f = 0; f < nf; ++f) {
+      const int k = c->h_elface[(size_t)e * nf + f].x;
+      if (k >= 0 && k < e) lv = std::max(lv, fl[k] + 1);
+    }
+    fl[e] = lv;
+    fmax = std::max(fmax, lv);
+  }
+  for (int e = nel - 1; e >= 0; --e) {
+    int lv = 0;
+    for (int f = 0; f < nf; ++f) {
+      const int k = c->h_elface[(size_t)e * nf + f].x;
+      if (k > e) lv = std::max(lv, bl[k] + 1);
+    }
+    bl[e] = lv;
+    bmax = std::max(bmax, lv);
+  }
+  auto bucket = [&](const std::vector<int32_t>& lv, int nl, std::vector<int64_t>& ptr, std::vector<int32_t>& el) {
+    ptr.assign(nl + 2, 0);
+    for (int e = 0; e < nel; ++e) ptr[lv[e] + 1]++;
+    for (int l = 0; l <= nl; ++l) ptr[l + 1] += ptr[l];
+    el.assign(nel, 0);
+    std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int e = 0; e < nel; ++e) el[pos[lv[e]]++] = e;
+  };
+  std::vector<int32_t> fel, bel;
+  bucket(fl, fmax, p->fptr, fel);
+  bucket(bl, bmax, p->bptr, bel);
+  if (hipMalloc((void**)&p->d_fel, std::max(1, nel) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_bel, std::max(1, nel) * sizeof(int32_t)) != hipSuccess) {
+    set_error("device allocation of the ILU level lists failed");
+    return PDES_ERR_NOMEM;
+  }
+  PDES_HIP(hipMemcpyAsync(p->d_fel, fel.data(), nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipMemcpyAsync(p->d_bel, bel.data(), nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  p->ilu_ready = true;
+  return PDES_OK;
+}
+
+template <int NP, int ND, int NS>
+static int ilu_factor(pdes_ctx* c) {
+  pdes_jacobian* j = c->jac;
+  pdes_precond* p = c->pc_obj;
+  for (size_t l = 0; l + 1 < p->fptr.size(); ++l) {
+    const int64_t cnt = p->fptr[l + 1] - p->fptr[l];
+    if (cnt == 0) continue;
+    const int32_t* lev = p->d_fel + p->fptr[l];
+    if (j->fp64)
+      hipLaunchKernelGGL((ilu_factor_level_kernel<NP, ND, NS, double>), dim3((unsigned)cnt), dim3(256), 0, c->stream,
+                         lev, j->d_rowptr, j->d_colind, (const double*)j->d_vals, c->d_elface, c->nfaces, c->d_perm,
+                         p->d_inv, j->d_err, c->d_el_inst, c->d_inst_active);
+    else
+      hipLaunchKernelGGL((ilu_factor_level_kernel<NP, ND, NS, float>), dim3((unsigned)cnt), dim3(256), 0, c->stream,
+                         lev, j->d_rowptr, j->d_colind, (const float*)j->d_vals, c->d_elface, c->nfaces, c->d_perm,
+                         p->d_inv, j->d_err, c->d_el_inst, c->d_inst_active);
+  }
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+template <int NP, int ND>
+static int ilu_apply(pdes_ctx* c, const double* b, double* x) {
+  pdes_jacobian* j = c->jac;
+  pdes_precond* p = c->pc_obj;
+  constexpr int EPB = 256 / (NP * ND);
+  for (int pass = 0; pass < 2; ++pass) {
+    const std::vector<int64_t>& ptr = pass == 0 ? p->fptr : p->bptr;
+    const int32_t* els = pass == 0 ? p->d_fel : p->d_bel;
+    for (size_t l = 0; l + 1 < ptr.size(); ++l) {
+      const int64_t cnt = ptr[l + 1] - ptr[l];
+      if (cnt == 0) continue;
+      const dim3 grid((unsigned)ceil_div(cnt, EPB));
+      if (j->fp64)
+        hipLaunchKernelGGL((ilu_sweep_level_kernel<NP, ND, double>), grid, dim3(256), 0, c->stream, els + ptr[l], cnt,
+                           j->d_rowptr, j->d_colind, (const double*)j->d_vals, p->d_inv, b, x, pass == 0,
+                           c->d_el_inst, c->d_mask, c->d_inst_active);
+      else
+        hipLaunchKernelGGL((ilu_sweep_level_kernel<NP, ND, float>), grid, dim3(256), 0, c->stream, els + ptr[l], cnt,
+                           j->d_rowptr, j->d_colind, (const float*)j->d_vals, p->d_inv, b, x, pass == 0,
+                           c->d_el_inst, c->d_mask, c->d_inst_active);
+    }
+  }
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
 template <int NP, int ND>
 static int bj_factor(pdes_ctx* c) {
   pdes_jacobian* j = c->jac;
@@ -222,7 +470,26 @@ __global__ void copy_kernel(const double* __restrict__ a, double* __restrict__ b
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
 }
 
+static int ilu_factor_dispatch(pdes_ctx* c) {
+#define PDES_ILU_F(D_, NP_, NF_, NS_, DE_) \
+  if (c->dim == D_ && c->Np == NP_ && (c->diag_e != 0) == DE_) return ilu_factor<NP_, D_ + 2, NS_>(c);
+  PDES_FOR_EACH_SHAPE(PDES_ILU_F)
+#undef PDES_ILU_F
+  set_error("no ILU(0) kernel for this operator");
+  return PDES_ERR_UNSUPPORTED;
+}
+
+static int ilu_apply_dispatch(pdes_ctx* c, const double* b, double* x) {
+#define PDES_ILU_A(D_, NP_, NF_, NS_, DE_) \
+  if (c->dim == D_ && c->Np == NP_ && (c->diag_e != 0) == DE_) return ilu_apply<NP_, D_ + 2>(c, b, x);
+  PDES_FOR_EACH_SHAPE(PDES_ILU_A)
+#undef PDES_ILU_A
+  set_error("no ILU(0) kernel for this operator");
+  return PDES_ERR_UNSUPPORTED;
+}
+
 int pc_apply(pdes_ctx* c, const double* b, double* x) {
+  if (c->pc_obj && c->pc_obj->kind == PDES_PC_ILU0) return ilu_apply_dispatch(c, b, x);
   if (!c->pc_obj || c->pc_obj->kind == PDES_PC_NONE) {
     const int64_t n = c->numDof;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)std::min<int64_t>(8192, ceil_div(n, 256))), dim3(256), 0,
@@ -252,7 +519,7 @@ int pc_apply(pdes_ctx* c, const double* b, double* x) {
 }
 
 int pc_setup(pdes_ctx* c, int kind) {
-  PDES_CHECK(kind == PDES_PC_NONE || kind == PDES_PC_BJACOBI, PDES_ERR_UNSUPPORTED,
+  PDES_CHECK(kind == PDES_PC_NONE || kind == PDES_PC_BJACOBI || kind == PDES_PC_ILU0, PDES_ERR_UNSUPPORTED,
              "preconditioner kind %d is not available", kind);
   if (!c->pc_obj) c->pc_obj = new pdes_precond();
   pdes_precond* p = c->pc_obj;
@@ -268,7 +535,13 @@ int pc_setup(pdes_ctx* c, int kind) {
       return PDES_ERR_NOMEM;
     }
   }
-  int st = pc_factor(c);
+  int st;
+  if (kind == PDES_PC_ILU0) {
+    if (!p->ilu_ready && (st = ilu_structure(c, p)) != 0) return st;
+    st = ilu_factor_dispatch(c);
+  } else {
+    st = pc_factor(c);
+  }
   if (st) return st;
   p->factored = true;
   return PDES_OK;
@@ -540,7 +813,7 @@ int pdes_pc_setup(pdes_ctx* c, int kind) {
     PDES_HIP(hipStreamSynchronize(c->stream));
     if (err) {
       PDES_HIP(hipMemsetAsync(c->jac->d_err, 0, sizeof(int32_t), c->stream));
-      set_error("singular element block in the block-Jacobi factorisation");
+      set_error("singular element block in the preconditioner factorisation");
       return PDES_ERR_PHYS;
     }
   }

@@ -140,6 +140,14 @@ class HIPBJacobiPC(AbstractPC):
         x[...] = out.reshape(x.shape)
 
 
+class HIPILU0PC(HIPBJacobiPC):
+    """Element-block ILU(0) in the natural element order, FP32 factors:
+    D'_e = A_ee - sum_{k<e neighbour} A_ek D'_k^-1 A_ke and
+    M = (D' + L) D'^-1 (D' + U); the device stand-in for the PETSc
+    bjacobi/ILU sub-preconditioner of the reference (read_input.jl:560-581)."""
+    kind = abi.PDES_PC_ILU0
+
+
 class StandardLinearSolver:
     """LinearSolver with a PC and an LO; linearSolve runs GMRES(restart)
     with right preconditioning on the device."""
