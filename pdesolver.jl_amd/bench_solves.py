@@ -68,8 +68,9 @@ def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_to
         e.set_q(q0)
         seq_reps.append(linsolve.newton(m, op, e, opts))
         seq_lat.append(time.perf_counter() - t1)
-        if len(seq_lat) % 16 == 0:
-            print(f"sequential {len(seq_lat)}/{count} solved", file=sys.stderr, flush=True)
+        print(f"sequential {len(seq_lat)}/{count}: {op.kind} p={op.degree} {m.numEl} elements, "
+              f"{seq_reps[-1].iterations} Newton / {seq_reps[-1].total_krylov_iters} Krylov, {seq_lat[-1]:.2f} s",
+              file=sys.stderr, flush=True)
     t_seq = time.perf_counter() - t0
     q_seq = [e.get_q() for e in ctxs]
     for e in ctxs:

@@ -1,6 +1,7 @@
 // ctx.hip -- context lifetime, table upload and state transfer (C ABI).
 #include <cstdarg>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "ctx.hpp"
@@ -52,6 +53,7 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   PDES_CHECK(m->dim == 2 || m->dim == 3, PDES_ERR_ARG, "dim must be 2 or 3");
   PDES_CHECK(m->numEl > 0 && s->Np > 0 && s->Nf > 0, PDES_ERR_ARG, "empty mesh or operator");
   pdes_ctx* c = new pdes_ctx();
+  c->use_graphs = std::getenv("PDES_NO_GRAPHS") == nullptr;
   c->dim = m->dim;
   c->nd = m->dim + 2;
   c->nfaces = m->dim + 1;

@@ -59,6 +59,9 @@ struct pdes_ctx {
   // packed batches: 0/1 per instance, 1 while the instance is in the Newton
   // work list (kernels that walk fixed element lists skip the others)
   int32_t* d_inst_active = nullptr;
+  // replay multi-launch sequences (ILU sweeps) from captured HIP graphs;
+  // off when the environment variable PDES_NO_GRAPHS is set at creation
+  bool use_graphs = true;
   int partial_n = 0;
   // residual launch variants (see residual_launch.hpp) for the plain and the
   // RK-epilogue kernel, chosen by the autotuner or the default table

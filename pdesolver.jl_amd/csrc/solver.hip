@@ -29,6 +29,18 @@ struct pdes_precond {
   std::vector<int64_t> fptr, bptr;
   int32_t* d_fel = nullptr;
   int32_t* d_bel = nullptr;
+  // the 2 x levels sweep launches of one apply, captured once per (mask,
+  // matrix) into a HIP graph over fixed staging vectors and replayed
+  double* d_stage_in = nullptr;
+  double* d_stage_out = nullptr;
+  struct Graph {
+    const int32_t* mask;
+    const void* vals;
+    bool fp64;
+    const float* inv;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
 };
 
 namespace pdes {
@@ -42,6 +54,9 @@ int precond_free(pdes_ctx* c) {
   if (c->pc_obj->d_inv) hipFree(c->pc_obj->d_inv);
   if (c->pc_obj->d_fel) hipFree(c->pc_obj->d_fel);
   if (c->pc_obj->d_bel) hipFree(c->pc_obj->d_bel);
+  if (c->pc_obj->d_stage_in) hipFree(c->pc_obj->d_stage_in);
+  if (c->pc_obj->d_stage_out) hipFree(c->pc_obj->d_stage_out);
+  for (auto& g : c->pc_obj->graphs) hipGraphExecDestroy(g.exec);
   delete c->pc_obj;
   c->pc_obj = nullptr;
   return PDES_OK;
@@ -54,30 +69,28 @@ int precond_free(pdes_ctx* c) {
 // coalesced matrix-vector product.
 // ---------------------------------------------------------------------------
 template <int N>
-__device__ void gauss_jordan_inverse(float (*A)[N + 1], int* perm, float* fcol, float* redv, int* redi, int* pivrow,
-                                     int32_t* err) {
+__device__ void gauss_jordan_inverse(float (*A)[N + 1], int* perm, float* fcol, int* pivrow, int32_t* err) {
   const int tid = threadIdx.x;
   for (int k = 0; k < N; ++k) {
-    // pivot: max |A[r][k]| over r >= k, lowest row on ties (first wave)
+    // pivot: max |A[r][k]| over r >= k, lowest row on ties (first wave,
+    // butterfly reduction of (value, row) pairs)
     if (tid < 64) {
       float best = -1.0f;
-      int bi = k;
+      int bi = N;
       for (int r = k + tid; r < N; r += 64) {
         const float v = fabsf(A[r][k]);
         if (v > best) { best = v; bi = r; }
       }
-      redv[tid] = best;
-      redi[tid] = bi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      float best = -1.0f;
-      int bi = k;
-      for (int t = 0; t < 64; ++t)
-        if (redv[t] > best || (redv[t] == best && redi[t] < bi)) { best = redv[t]; bi = redi[t]; }
-      *pivrow = bi;
-      perm[k] = bi;
-      if (!(best > 0.0f)) err[0] = 1;
+      for (int off = 32; off > 0; off >>= 1) {
+        const float ov = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (tid == 0) {
+        *pivrow = bi;
+        perm[k] = bi;
+        if (!(best > 0.0f)) err[0] = 1;
+      }
     }
     __syncthreads();
     const int p = *pivrow;
@@ -125,8 +138,6 @@ bjacobi_factor_kernel(const int32_t* __restrict__ el_list, const int64_t* __rest
   constexpr int N = NP * ND;
   __shared__ float A[N][N + 1];
   __shared__ float fcol[N];
-  __shared__ float redv[64];
-  __shared__ int redi[64];
   __shared__ int perm[N];
   __shared__ int pivrow;
   const int tid = threadIdx.x;
@@ -140,7 +151,7 @@ bjacobi_factor_kernel(const int32_t* __restrict__ el_list, const int64_t* __rest
         A[i * ND + a][j * ND + b] = pos >= 0 ? (float)vals[pos * ND * ND + a * ND + b] : 0.0f;
   }
   __syncthreads();
-  gauss_jordan_inverse<N>(A, perm, fcol, redv, redi, &pivrow, err);
+  gauss_jordan_inverse<N>(A, perm, fcol, &pivrow, err);
   float* out = inv + e * N * N;
   for (int t = tid; t < N * N; t += blockDim.x) {
     const int cc = t / N, r = t - cc * N;
@@ -173,7 +184,7 @@ bjacobi_apply_kernel(const int32_t* __restrict__ el_list, int64_t nel, const flo
   if (!act) return;
   const float* M = inv + e * N * N;
   float acc = 0.0f;
-#pragma unroll 4
+#pragma unroll
   for (int cc = 0; cc < N; ++cc) acc += M[cc * N + r] * v[slot][cc];
   x[e * N + r] = (double)acc;
 }
@@ -204,8 +215,6 @@ ilu_factor_level_kernel(const int32_t* __restrict__ lev_el, const int64_t* __res
   __shared__ float B2[M][M + 1];
   __shared__ float B3[M][M + 1];
   __shared__ float fcol[N];
-  __shared__ float redv[64];
-  __shared__ int redi[64];
   __shared__ int permk[N];
   __shared__ int pivrow;
   __shared__ int Ce[NS], Rk[NS];
@@ -264,7 +273,7 @@ ilu_factor_level_kernel(const int32_t* __restrict__ lev_el, const int64_t* __res
     }
     __syncthreads();
   }
-  gauss_jordan_inverse<N>(A, permk, fcol, redv, redi, &pivrow, err);
+  gauss_jordan_inverse<N>(A, permk, fcol, &pivrow, err);
   float* out = inv + e * N * N;
   for (int t = tid; t < N * N; t += blockDim.x) {
     const int cc = t / N, r = t - cc * N;
@@ -300,16 +309,32 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
     const int i = r / ND, a = r - i * ND;
     const int64_t row = e * NP + i;
     double acc = forward ? b[e * N + r] : 0.0;
-    for (int64_t p = rowptr[row]; p < rowptr[row + 1]; ++p) {
-      const int64_t col = colind[p];
-      const int64_t ce = col / NP;
-      if (forward ? (ce >= e) : (ce <= e)) continue;
-      const BT* blk = vals + p * ND * ND + a * ND;
-      const double* xc = x + col * ND;
-      double t = 0.0;
+    // columns are sorted: lower-element blocks precede the element's own run
+    // of NP blocks, upper-element blocks follow it
+    const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
+    const int32_t own0 = (int32_t)(e * NP), own1 = (int32_t)(e * NP + NP);
+    if (forward) {
+      for (int64_t p = p0; p < p1; ++p) {
+        const int32_t col = colind[p];
+        if (col >= own0) break;
+        const BT* blk = vals + p * ND * ND + a * ND;
+        const double* xc = x + (int64_t)col * ND;
+        double t = 0.0;
 #pragma unroll
-      for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
-      acc = forward ? acc - t : acc + t;
+        for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
+        acc -= t;
+      }
+    } else {
+      for (int64_t p = p1 - 1; p >= p0; --p) {
+        const int32_t col = colind[p];
+        if (col < own1) break;
+        const BT* blk = vals + p * ND * ND + a * ND;
+        const double* xc = x + (int64_t)col * ND;
+        double t = 0.0;
+#pragma unroll
+        for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
+        acc += t;
+      }
     }
     v[slot][r] = (float)acc;
   }
@@ -317,7 +342,7 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
   if (!act) return;
   const float* Mi = inv + e * N * N;
   float y = 0.0f;
-#pragma unroll 4
+#pragma unroll
   for (int cc = 0; cc < N; ++cc) y += Mi[cc * N + r] * v[slot][cc];
   if (forward) x[e * N + r] = (double)y;
   else x[e * N + r] -= (double)y;
@@ -488,8 +513,52 @@ static int ilu_apply_dispatch(pdes_ctx* c, const double* b, double* x) {
   return PDES_ERR_UNSUPPORTED;
 }
 
+// ILU(0) apply through a captured graph: b is copied into the staging input,
+// the graph (all level launches) is replayed, the staging output is copied to
+// x.  Falls back to direct launches if the capture is refused.
+static int ilu_apply_graphed(pdes_ctx* c, const double* b, double* x) {
+  pdes_precond* p = c->pc_obj;
+  const size_t bytes = (size_t)c->numDof * sizeof(double);
+  if (!p->d_stage_in) {
+    if (hipMalloc((void**)&p->d_stage_in, bytes) != hipSuccess ||
+        hipMalloc((void**)&p->d_stage_out, bytes) != hipSuccess) {
+      set_error("ILU staging allocation failed");
+      return PDES_ERR_NOMEM;
+    }
+  }
+  hipGraphExec_t exec = nullptr;
+  for (auto& g : p->graphs)
+    if (g.mask == c->d_mask && g.vals == c->jac->d_vals && g.fp64 == c->jac->fp64 && g.inv == p->d_inv)
+      exec = g.exec;
+  if (!exec) {
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      return ilu_apply_dispatch(c, b, x);
+    }
+    const int st = ilu_apply_dispatch(c, p->d_stage_in, p->d_stage_out);
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (st) {
+      if (graph) hipGraphDestroy(graph);
+      return st;
+    }
+    if (ec != hipSuccess || !graph || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      return ilu_apply_dispatch(c, b, x);
+    }
+    hipGraphDestroy(graph);
+    p->graphs.push_back({c->d_mask, c->jac->d_vals, c->jac->fp64, p->d_inv, exec});
+  }
+  PDES_HIP(hipMemcpyAsync(p->d_stage_in, b, bytes, hipMemcpyDeviceToDevice, c->stream));
+  PDES_HIP(hipGraphLaunch(exec, c->stream));
+  PDES_HIP(hipMemcpyAsync(x, p->d_stage_out, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return PDES_OK;
+}
+
 int pc_apply(pdes_ctx* c, const double* b, double* x) {
-  if (c->pc_obj && c->pc_obj->kind == PDES_PC_ILU0) return ilu_apply_dispatch(c, b, x);
+  if (c->pc_obj && c->pc_obj->kind == PDES_PC_ILU0)
+    return c->use_graphs ? ilu_apply_graphed(c, b, x) : ilu_apply_dispatch(c, b, x);
   if (!c->pc_obj || c->pc_obj->kind == PDES_PC_NONE) {
     const int64_t n = c->numDof;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)std::min<int64_t>(8192, ceil_div(n, 256))), dim3(256), 0,
@@ -529,6 +598,8 @@ int pc_setup(pdes_ctx* c, int kind) {
   const int n = c->Np * c->nd;
   if (!p->d_inv || p->n != n) {
     if (p->d_inv) hipFree(p->d_inv);
+    for (auto& g : p->graphs) hipGraphExecDestroy(g.exec);
+    p->graphs.clear();
     p->n = n;
     if (hipMalloc((void**)&p->d_inv, (size_t)c->numEl * n * n * sizeof(float)) != hipSuccess) {
       set_error("device allocation of the block-Jacobi factors failed");
