@@ -104,6 +104,14 @@ typedef struct {
 
 typedef struct pdes_ctx pdes_ctx;
 
+/* estimateMem (src/estimateMem.jl:1-74) for the device: bytes of HBM that one
+ * steady solve of this mesh/operator allocates (context, BSR Jacobian in FP32
+ * or FP64 blocks, preconditioner of kind pc_kind, GMRES(gmres_restart) basis,
+ * Newton vectors; batched adds the per-instance share of a packed batch).
+ * Used for admission by footprint. */
+int pdes_estimate_bytes(const pdes_mesh_desc* mesh, const pdes_sbp_desc* sbp, int fp64_blocks,
+                        int pc_kind, int gmres_restart, int batched, int64_t* bytes);
+
 const char* pdes_last_error(void);
 int pdes_version(void);
 
@@ -145,6 +153,8 @@ int pdes_time_residual(pdes_ctx* ctx, int iters, double* avg_ms);
  * (pdes_copy_kernel) `iters` times; avg_ms receives the average kernel time.
  * Used as the attainable-bandwidth reference and to scale HBM byte counters. */
 int pdes_copy_bandwidth(int64_t nbytes, int iters, double* avg_ms);
+/* Free and total device memory in bytes (hipMemGetInfo), for footprint checks. */
+int pdes_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 int pdes_rk4(pdes_ctx* ctx, int nsteps, double h, double* elapsed_ms);
 int pdes_jv_device(pdes_ctx* ctx, const double* v_dev, double* b_dev);
 int pdes_sync(pdes_ctx* ctx);

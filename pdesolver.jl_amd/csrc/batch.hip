@@ -632,11 +632,14 @@ int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
           s.rep.step_norm <= o->step_tol) {
         s.rep.converged = 1;
         s.active = false;
+        // retired between Newton steps: its result is final from here on
+        s.rep.elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
       }
     }
   }
   const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  for (auto& s : g->inst) s.rep.elapsed_s = el;
+  for (auto& s : g->inst)
+    if (s.rep.converged != 1) s.rep.elapsed_s = el;
   c->nlist = -1;
   c->ptc_on = false;
   return PDES_OK;

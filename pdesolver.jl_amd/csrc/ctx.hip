@@ -222,6 +222,42 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   return PDES_OK;
 }
 
+// estimateMem (src/estimateMem.jl:1-74) for the device: the bytes one steady
+// solve allocates, summed from the mesh and operator counts exactly as the
+// allocations are sized (context arrays, BSR Jacobian, preconditioner, GMRES
+// basis and Newton vectors; the per-instance share of a packed batch when
+// batched is set).  The metric is counted per node (the compact per-element
+// form is only known after the bitwise affinity check).
+int pdes_estimate_bytes(const pdes_mesh_desc* m, const pdes_sbp_desc* s, int fp64_blocks, int pc_kind,
+                        int gmres_restart, int batched, int64_t* bytes) {
+  PDES_CHECK(m && s && bytes && gmres_restart > 0, PDES_ERR_ARG, "bad argument");
+  const int64_t dim = m->dim, nd = dim + 2, nfaces = dim + 1, Np = s->Np, Nf = s->Nf;
+  const int64_t nel = m->numEl, nI = m->numInterfaces, nB = m->numBoundaryFaces;
+  const int64_t ndof = nel * Np * nd;
+  int64_t b = 0;
+  // context
+  b += nel * Np * dim * dim * 8;                       // dxidx
+  b += nel * nfaces * 16;                              // element -> face records
+  b += (nI + 2 * nB) * Nf * dim * 8;                   // face normals, boundary normals and coordinates
+  b += (dim * Np * Np + s->stencil * Nf + Nf) * 8 + (s->stencil * nfaces + Nf * s->norient + Np * nfaces) * 4;
+  b += ndof * 8 * 5;                                   // Minv, q, res, two work vectors
+  b += nel * Np * 8;                                   // implicit-Euler diagonal
+  b += (4096 + 64 + 8) * 8;                            // reduction scratch, scalars, flags
+  // BSR Jacobian: element blocks plus the face couplings of every interface
+  const int64_t cpl = s->diag_e ? Nf : (int64_t)s->stencil * s->stencil;
+  const int64_t nnzb = nel * Np * Np + 2 * nI * cpl;
+  b += (nel * Np + 1) * 8 + nnzb * 4 + nnzb * nd * nd * (fp64_blocks ? 8 : 4) + 4;
+  // preconditioner: one FP32 inverse per element (block-Jacobi and ILU(0))
+  if (pc_kind != PDES_PC_NONE) b += nel * (Np * nd) * (Np * nd) * 4;
+  if (pc_kind == PDES_PC_ILU0) b += 2 * nel * 4 + 2 * ndof * 8;   // level lists, graph staging
+  // GMRES basis and vectors, Newton step and right-hand side
+  b += (gmres_restart + 1) * ndof * 8 + 2 * ndof * 8 + (int64_t)(gmres_restart + 2) * 512 * 8;
+  b += 2 * ndof * 8;
+  if (batched) b += 2 * nel * 4 + 2 * ndof * 8 + (ndof / 4096 + 1) * (8 + 4 + 4 + 64 * 8);
+  *bytes = b;
+  return PDES_OK;
+}
+
 int pdes_ctx_destroy(pdes_ctx* c) {
   if (!c) return PDES_OK;
   if (c->stream) hipStreamSynchronize(c->stream);

@@ -193,6 +193,15 @@ int pdes_copy_bandwidth(int64_t nbytes, int iters, double* avg_ms) {
   return st;
 }
 
+int pdes_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  PDES_CHECK(free_bytes && total_bytes, PDES_ERR_ARG, "null argument");
+  size_t f = 0, t = 0;
+  PDES_HIP(hipMemGetInfo(&f, &t));
+  *free_bytes = (int64_t)f;
+  *total_bytes = (int64_t)t;
+  return PDES_OK;
+}
+
 int pdes_time_residual(pdes_ctx* c, int iters, double* avg_ms) {
   PDES_CHECK(c && avg_ms && iters > 0, PDES_ERR_ARG, "bad argument");
   hipEvent_t a, b;

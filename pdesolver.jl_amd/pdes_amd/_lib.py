@@ -36,6 +36,8 @@ _SIGS = {
     "pdes_jv_device": [_vp, _vp, _vp],
     "pdes_time_residual": [_vp, C.c_int, _dp],
     "pdes_copy_bandwidth": [C.c_int64, C.c_int, _dp],
+    "pdes_estimate_bytes": [_vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int64)],
+    "pdes_device_memory": [C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pdes_autotune": [_vp, C.c_int, _i32p],
     "pdes_set_launch_variant": [_vp, C.c_int, C.c_int],
     "pdes_rk4": [_vp, C.c_int, C.c_double, _dp],

@@ -1,0 +1,78 @@
+"""Served solves: admission by HBM footprint, concurrent lanes, and results
+that do not depend on what a request was batched or overlapped with."""
+import numpy as np
+import pytest
+
+from pdes_amd import abi, linsolve, problems
+from pdes_amd.batch import BatchSolver
+from pdes_amd.serve import SolveServer, device_memory, estimate_bytes
+
+pytestmark = pytest.mark.gpu
+
+OPTS = linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=30, pc_kind=abi.PDES_PC_ILU0)
+
+
+def _requests():
+    rng = np.random.default_rng(21)
+    out = []
+    for i in range(8):
+        p = 1 + i % 2
+        Ma = float(rng.uniform(0.3, 0.8))
+        if i % 3 != 2:
+            m, op, ph = problems.make_problem("vortex", int(rng.integers(4, 9)), p, "omega", "roe", vortex_Min=Ma)
+            q0 = problems.vortex_noise_state(m, ph, seed=i)
+        else:
+            m, op, ph = problems.make_problem("cube", 2, p, "omega", "roe", Ma=Ma, aoa_deg=float(rng.uniform(-5, 5)))
+            q0 = problems.freestream_bump_state(m, ph, seed=i)
+        out.append((m, op, ph, q0))
+    return out
+
+
+def test_served_results_equal_solo_solves():
+    """Requests queue behind a budget of about two requests, are admitted as
+    memory frees up, overlap on two lanes, and each result equals (bit for
+    bit) the same request solved alone."""
+    reqs = _requests()
+    sizes = [estimate_bytes(m, op, ph, OPTS) for (m, op, ph, q0) in reqs]
+    budget = 2 * max(sizes)
+    srv = SolveServer(budget, lanes=2, opts=OPTS)
+    tickets = [srv.submit(m, op, ph, q0) for (m, op, ph, q0) in reqs]
+    for t in tickets:
+        assert t.wait(120.0), "request did not finish"
+    srv.shutdown()
+    assert srv.peak_bytes <= budget
+    for i, (t, (m, op, ph, q0)) in enumerate(zip(tickets, reqs)):
+        assert t.error is None, t.error
+        print(i, m.dim, op.degree, m.numEl, "batch", t.batch_size, "it", t.report.iterations, "norm", t.report.res_norm,
+              flush=True)
+        assert t.report.converged == 1 and t.report.res_norm <= 1e-10
+        solo = BatchSolver([(m, op, ph)])
+        solo.set_state(0, q0)
+        r = solo.solve(OPTS)[0]
+        assert r.iterations == t.report.iterations
+        assert np.array_equal(solo.get_state(0), t.q), f"request {i} differs from its solo solve"
+        solo.free()
+
+
+def test_request_above_budget_is_rejected():
+    m, op, ph = problems.make_problem("vortex", 4, 1, "omega", "roe")
+    srv = SolveServer(1024, lanes=1, opts=OPTS)
+    with pytest.raises(ValueError):
+        srv.submit(m, op, ph, problems.vortex_noise_state(m, ph))
+    srv.shutdown()
+
+
+def test_footprint_estimate_covers_the_allocations():
+    """pdes_estimate_bytes against the device memory a batch really takes
+    (free memory before creation minus free memory after a solve)."""
+    m, op, ph = problems.make_problem("cube", 6, 2, "omega", "roe", Ma=0.5, aoa_deg=2.0)
+    est = estimate_bytes(m, op, ph, OPTS)
+    free0, _ = device_memory()
+    bs = BatchSolver([(m, op, ph)])
+    bs.set_state(0, problems.freestream_bump_state(m, ph))
+    bs.solve(OPTS)
+    free1, _ = device_memory()
+    bs.free()
+    used = free0 - free1
+    print("estimate", est, "measured", used, "ratio", used / est)
+    assert 0.6 * est <= used <= 1.15 * est
