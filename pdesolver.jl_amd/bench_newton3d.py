@@ -1,0 +1,82 @@
+"""3D Newton-Krylov, mixed precision: the unit cube meshed as n^3 hexes split
+into 6 tetrahedra each (n = 32: 196,608 elements), SBP-Omega p=2 with the Roe
+flux and freestream BCs (rho = 1, M = 0.5, 2 degrees angle of attack), initial
+state freestream plus a seeded smooth 1e-2 bump.  Residual, Newton and
+GMRES(30) in FP64; Jacobian blocks and the preconditioner in FP32; the same
+solve with FP64 blocks for comparison, to ||R|| <= 1e-10.
+
+    python pdesolver.jl_amd/bench_newton3d.py [--n 32] [--pc ilu0]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import numpy as np  # noqa: E402
+
+from pdes_amd import abi, euler, linsolve, problems  # noqa: E402
+
+PCS = {"bjacobi": abi.PDES_PC_BJACOBI, "ilu0": abi.PDES_PC_ILU0}
+
+
+def run(n=32, p=2, pc="ilu0", krylov_reltol=1e-2):
+    t0 = time.perf_counter()
+    m, op, ph = problems.make_problem("cube", n, p, "omega", "roe", Ma=0.5, aoa_deg=2.0)
+    q0 = problems.freestream_bump_state(m, ph, seed=2)
+    setup = time.perf_counter() - t0
+    out = {"workload": f"unit cube {n}^3 x 6 tets ({m.numEl} elements), Omega p={p}, Roe, freestream M=0.5 aoa=2 deg, "
+                       f"Newton to ||R|| <= 1e-10, GMRES(30) rtol {krylov_reltol}, {pc}",
+           "numDof": int(q0.size), "setup_s": setup}
+    states = {}
+    for fp64 in (False, True):
+        eqn = euler.EulerData(m, op, ph)
+        eqn.set_q(q0)
+        opts = linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=krylov_reltol, pc_kind=PCS[pc],
+                                      fp64_blocks=fp64)
+        # one Newton step per call so that every step reports its progress;
+        # the device state carries over, so this is newtonInner step by step
+        opts.itermax = 1
+        t1 = time.perf_counter()
+        its = kry = 0
+        norm0 = None
+        rep = None
+        for _ in range(50):
+            rep = linsolve.newton(m, op, eqn, opts)
+            norm0 = rep.res_norm0 if norm0 is None else norm0
+            its += rep.iterations
+            kry += rep.total_krylov_iters
+            print(f"fp64_blocks={fp64} step {its}: ||R|| {rep.res_norm:.3e}, {rep.total_krylov_iters} Krylov, "
+                  f"{time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
+            if rep.converged:
+                break
+        dt = time.perf_counter() - t1
+        states[fp64] = eqn.get_q()
+        out["fp64_blocks" if fp64 else "mixed"] = {
+            "converged": int(rep.converged), "newton_iterations": its, "krylov_iterations": kry,
+            "res_norm0": norm0, "res_norm": rep.res_norm, "solve_s": dt}
+        eqn.free()
+    d = states[False] - states[True]
+    w = (op.w[None, :] / m.jac)[:, :, None]          # H-norm weights (mass matrix)
+    out["max_abs_state_diff"] = float(np.abs(d).max())
+    out["H_norm_state_diff"] = float(np.sqrt((d * d * w).sum()))
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=32)
+    ap.add_argument("--p", type=int, default=2)
+    ap.add_argument("--pc", choices=sorted(PCS), default="ilu0")
+    a = ap.parse_args()
+    print(json.dumps(run(a.n, a.p, a.pc)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
