@@ -819,6 +819,12 @@ int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_de
       set_error("batch upload failed");
       st = PDES_ERR_HIP;
     }
+    // boundary states with each instance's own physics (records now tagged)
+    if (!st) st = bc_state_setup(c);
+    if (!st && hipStreamSynchronize(s) != hipSuccess) {
+      set_error("boundary state setup failed");
+      st = PDES_ERR_HIP;
+    }
   }
   if (st) {
     for (auto* g : B->groups) group_free(g);

@@ -38,6 +38,21 @@ int sync_check_flag(pdes_ctx* c) {
   return PDES_OK;
 }
 
+// boundary states of all boundary face nodes into d_qbc (re-run whenever the
+// element-face records or the per-instance physics change)
+int bc_state_setup(pdes_ctx* c) {
+  const int64_t n = (int64_t)c->numEl * c->nfaces * c->Nf;
+  if (n == 0 || !c->d_qbc) return PDES_OK;
+  const dim3 grid((unsigned)ceil_div(n, 256));
+  if (c->dim == 2)
+    hipLaunchKernelGGL(bc_state_kernel<2>, grid, dim3(256), 0, c->stream, c->devmesh(), c->pc, c->nfaces, c->Nf,
+                       c->d_qbc);
+  else
+    hipLaunchKernelGGL(bc_state_kernel<3>, grid, dim3(256), 0, c->stream, c->devmesh(), c->pc, c->nfaces, c->Nf,
+                       c->d_qbc);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
 }  // namespace pdes
 
 using namespace pdes;
@@ -213,6 +228,8 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   if (!st) st = upload<double>(&c->d_scalar, nullptr, 64, sm);
   if (!st) st = upload<int32_t>(&c->d_flag, nullptr, 1, sm);
   if (!st && hipMemsetAsync(c->d_flag, 0, sizeof(int32_t), sm) != hipSuccess) st = PDES_ERR_HIP;
+  if (!st) st = upload<double>(&c->d_qbc, nullptr, std::max<size_t>(1, nB * Nf * c->nd), sm);
+  if (!st) st = bc_state_setup(c);
   if (!st && hipStreamSynchronize(sm) != hipSuccess) { set_error("upload failed"); st = PDES_ERR_HIP; }
   if (st) {
     pdes_ctx_destroy(c);
@@ -267,7 +284,7 @@ int pdes_ctx_destroy(pdes_ctx* c) {
                   c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_q,
                   c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
                   c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale,
-                  c->d_inst_active};
+                  c->d_inst_active, c->d_qbc};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);

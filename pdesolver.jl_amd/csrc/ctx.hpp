@@ -59,6 +59,7 @@ struct pdes_ctx {
   // packed batches: 0/1 per instance, 1 while the instance is in the Newton
   // work list (kernels that walk fixed element lists skip the others)
   int32_t* d_inst_active = nullptr;
+  double* d_qbc = nullptr;   // boundary states [numBoundaryFaces][Nf][ndof]
   // replay multi-launch sequences (ILU sweeps) from captured HIP graphs;
   // off when the environment variable PDES_NO_GRAPHS is set at creation
   bool use_graphs = true;
@@ -73,7 +74,7 @@ struct pdes_ctx {
 
   pdes::DevMesh devmesh() const {
     return pdes::DevMesh{numEl, metric_per_node, nlist >= 0 ? d_el_list : nullptr, nlist, d_el_inst, d_inst_phys,
-                         d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry};
+                         d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry, d_qbc};
   }
   // elements processed by a launch (the work list when one is set)
   int64_t active_count() const { return nlist >= 0 ? nlist : numEl; }
@@ -92,4 +93,5 @@ int launch_wnorm2(pdes_ctx* c, const double* x, const double* w, int64_t n, doub
 int sync_check_flag(pdes_ctx* c);
 int jacobian_free(pdes_ctx* c);
 int precond_free(pdes_ctx* c);
+int bc_state_setup(pdes_ctx* c);
 }  // namespace pdes

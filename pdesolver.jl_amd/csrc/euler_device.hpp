@@ -207,7 +207,8 @@ __device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* q
   if constexpr (DIM == 3) k1 = k1 + qc[3] * qc[3];
   k1 = 0.5 * k1 / qc[0];
   T rho_int = qc[ND - 1] - k1;
-  T s = dlog(gamma_1 * rho_int / dpow(qc[0], gamma));
+  // log(gamma_1 rho_int / rho^gamma) taken as log(gamma_1 rho_int) - gamma log(rho)
+  T s = dlog(gamma_1 * rho_int) - gamma * dlog(qc[0]);
   T fac = 1.0 / rho_int;
   T tmp1 = -qc[0] * fac * gamma_1i;
   qe[0] = ((rho_int * (gamma + 1.0 - s) - qc[ND - 1]) * fac) * gamma_1i;
@@ -227,57 +228,45 @@ __device__ __forceinline__ void irslf_flux(const PhysConst& p, const T* qL, cons
   ir_zvars<DIM>(p, qL, zL);
   ir_zvars<DIM>(p, qR, zR);
   ir_flux_z<DIM, 1>(p, zL, zR, nrm, F);
-  T qa[ND], vL[ND], vR[ND];
-#pragma unroll
-  for (int i = 0; i < ND; ++i) qa[i] = 0.5 * (qL[i] + qR[i]);
-  to_ir_vars<DIM>(p, qL, vL);
+  // entropy-variable jump dv = v(qL) - v(qR)
+  T dv[ND], vR[ND];
+  to_ir_vars<DIM>(p, qL, dv);
   to_ir_vars<DIM>(p, qR, vR);
 #pragma unroll
-  for (int i = 0; i < ND; ++i) vL[i] = vL[i] - vR[i];
-  // A0 = dq/dv at q_avg (getIRA0), lambda_max at q_avg
+  for (int i = 0; i < ND; ++i) dv[i] = dv[i] - vR[i];
+  T qa[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) qa[i] = 0.5 * (qL[i] + qR[i]);
+  // getLambdaMax(params, q_avg, dir) and A0 = dq/dv at q_avg (getIRA0),
+  // applied to dv in factored form: with m the momentum, E the energy,
+  // h = (E + p)/rho and s = m . dv_m,
+  //   row 0:    rho dv_0 + s + E dv_E
+  //   row d:    m_d (dv_0 + s/rho + h dv_E) + p dv_d
+  //   row E:    E dv_0 + h s + (rho h^2 - a^2 p/(gamma-1)) dv_E
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
   T pr = pressure<DIM>(p, qa);
   T rho = qa[0];
   T rhoinv = 1.0 / rho;
   T h = (qa[ND - 1] + pr) * rhoinv;
   T a2 = gamma * pr * rhoinv;
-  T A0[ND][ND];  // A0[row][col]
-  A0[0][0] = rho;
-#pragma unroll
-  for (int r = 1; r < ND; ++r) { A0[r][0] = qa[r]; A0[0][r] = qa[r]; }
-#pragma unroll
-  for (int r = 1; r <= DIM; ++r)
-#pragma unroll
-    for (int c = 1; c <= DIM; ++c) {
-      if (r == c) A0[r][c] = qa[r] * qa[r] * rhoinv + pr;
-      else if (c == 1 && r == 2) A0[r][c] = qa[1] * qa[2] * rhoinv;     // A0[3,2] 2D: rhou*rhov*rhoinv
-      else if (r == 1 && c == 2) A0[r][c] = qa[1] * qa[2] / rho;        // A0[2,3]: rhou*rhov/rho
-      else if (c == 1) A0[r][c] = qa[1] * qa[r] * rhoinv;               // rhou*rhow*rhoinv
-      else if (r == 1) A0[r][c] = qa[c] * qa[1] * rhoinv;               // rhow*rhou*rhoinv
-      else if (c == 2) A0[r][c] = qa[2] * qa[r] * rhoinv;               // rhov*rhow*rhoinv
-      else A0[r][c] = qa[c] * qa[r] * rhoinv;                           // rhow*rhov*rhoinv
-    }
-#pragma unroll
-  for (int d = 1; d <= DIM; ++d) { A0[ND - 1][d] = qa[d] * h; A0[d][ND - 1] = h * qa[d]; }
-  A0[ND - 1][ND - 1] = rho * h * h - a2 * pr / gamma_1;
-  // getLambdaMax(params, q_avg, dir)
   T Un = 0.0;
   double dA = 0.0;
-  T aL = dsqrt(gamma * pr * rhoinv);
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
     Un = Un + nrm[i] * qa[i + 1] * rhoinv;
     dA += nrm[i] * nrm[i];
   }
   dA = sqrt(dA);
-  T lambda_max = absvalue3(Un) + dA * aL;
+  const T lambda_max = absvalue3(Un) + dA * dsqrt(a2);
+  T sm = 0.0;
 #pragma unroll
-  for (int r = 0; r < ND; ++r) {
-    T acc = 0.0;
+  for (int d = 1; d <= DIM; ++d) sm = sm + qa[d] * dv[d];
+  const T dvE = dv[ND - 1];
+  const T common = dv[0] + sm * rhoinv + h * dvE;
+  F[0] = F[0] + lambda_max * (rho * dv[0] + sm + qa[ND - 1] * dvE);
 #pragma unroll
-    for (int c = 0; c < ND; ++c) acc = acc + A0[r][c] * vL[c];
-    F[r] = F[r] + acc * lambda_max;
-  }
+  for (int d = 1; d <= DIM; ++d) F[d] = F[d] + lambda_max * (qa[d] * common + pr * dv[d]);
+  F[ND - 1] = F[ND - 1] + lambda_max * (qa[ND - 1] * dv[0] + h * sm + (rho * h * h - a2 * pr / gamma_1) * dvE);
 }
 
 // calcIsentropicVortex  src/solver/euler/common_funcs.jl:25-76 (inflow Mach a parameter)

@@ -127,12 +127,7 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
         qL[c].d[c] = 1.0;
       }
       double qg[ND];
-      if constexpr (DIM == 2) {
-        if (bc_kind(r) == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(bc_phys(mesh, phys, r), mesh.coords_bndry + ((int64_t)r.y * NF + k) * DIM, qg);
-        else free_stream<DIM>(bc_phys(mesh, phys, r), qg);
-      } else {
-        free_stream<DIM>(bc_phys(mesh, phys, r), qg);
-      }
+      bc_state<DIM>(mesh, r, k, NF, qg);
 #pragma unroll
       for (int c = 0; c < ND; ++c) qR[c] = dual<2 * ND>(qg[c]);
       roe_flux<DIM>(phys, qL, qR, mesh.nrm_bndry + ((int64_t)r.y * NF + k) * DIM, F);

@@ -45,6 +45,7 @@ struct DevMesh {
   const double* __restrict__ nrm_face;    // [iface][k][dim]
   const double* __restrict__ nrm_bndry;   // [b][k][dim]
   const double* __restrict__ coords_bndry;// [b][k][dim]
+  const double* __restrict__ qbc;         // [b][k][ndof] boundary states (functions of the coordinates only)
 };
 
 struct DevSBP {
@@ -67,6 +68,16 @@ template <class P>
 __device__ __forceinline__ P bc_phys(const DevMesh& m, const P& phys, const ElFace& r) {
   if (m.inst_phys) return m.inst_phys[r.w >> 8];
   return phys;
+}
+
+// boundary state of face node k of boundary face rec.y, as stored by
+// bc_state_kernel (the BC state depends only on the coordinates and the
+// instance's physics, never on q)
+template <int DIM>
+__device__ __forceinline__ void bc_state(const DevMesh& m, const ElFace& rec, int k, int nf, double* qg) {
+  const double* s = m.qbc + ((int64_t)rec.y * nf + k) * (DIM + 2);
+#pragma unroll
+  for (int c = 0; c < DIM + 2; ++c) qg[c] = s[c];
 }
 
 // the DIM x DIM metric block [x][xi] of node j of element e
@@ -336,12 +347,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
           for (int c = 0; c < ND; ++c) qL[c] = sm.q[el][node][c];
           double qg[ND];
-          if constexpr (DIM == 2) {
-            if (bc_kind(rec) == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(bc_phys(mesh, phys, rec), mesh.coords_bndry + ((int64_t)rec.y * NF + k) * DIM, qg);
-            else free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
-          } else {
-            free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
-          }
+          bc_state<DIM>(mesh, rec, k, NF, qg);
 #pragma unroll
           for (int c = 0; c < ND; ++c) qR[c] = qg[c];
           roe_flux<DIM>(phys, qL, qR, nrm, Fk);
@@ -399,12 +405,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         }
         const int64_t b = rec.y;
         double qg[ND];
-        if constexpr (DIM == 2) {
-          if (bc_kind(rec) == PDES_BC_ISENTROPIC_VORTEX) isentropic_vortex(bc_phys(mesh, phys, rec), mesh.coords_bndry + (b * NF + k) * DIM, qg);
-          else free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
-        } else {
-          free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
-        }
+        bc_state<DIM>(mesh, rec, k, NF, qg);
 #pragma unroll
         for (int c = 0; c < ND; ++c) qR[c] = qg[c];
         roe_flux<DIM>(phys, qL, qR, mesh.nrm_bndry + (b * NF + k) * DIM, Fk);
@@ -512,6 +513,31 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       for (int c = 0; c < ND; ++c) Io<T>::store(res, jv, (e * NP + i) * ND + c, ri[c]);
     }
   }
+}
+
+// boundary states of every boundary face node: isentropicVortexBC
+// (bc.jl:554-567, calcIsentropicVortex) or FreeStreamBC (bc.jl:1573-1586),
+// with the instance's physics; one thread per (element face, face node)
+template <int DIM>
+__global__ void bc_state_kernel(DevMesh mesh, PhysConst phys, int nfaces, int nf, double* __restrict__ qbc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)mesh.numEl * nfaces * nf) return;
+  const int64_t ef = t / nf;
+  const int k = (int)(t - ef * nf);
+  const ElFace rec = mesh.elface[ef];
+  if (rec.x >= 0) return;
+  double qg[DIM + 2];
+  if constexpr (DIM == 2) {
+    if (bc_kind(rec) == PDES_BC_ISENTROPIC_VORTEX)
+      isentropic_vortex(bc_phys(mesh, phys, rec), mesh.coords_bndry + ((int64_t)rec.y * nf + k) * DIM, qg);
+    else
+      free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
+  } else {
+    free_stream<DIM>(bc_phys(mesh, phys, rec), qg);
+  }
+  double* out = qbc + ((int64_t)rec.y * nf + k) * (DIM + 2);
+#pragma unroll
+  for (int c = 0; c < DIM + 2; ++c) out[c] = qg[c];
 }
 
 }  // namespace pdes

@@ -34,10 +34,10 @@ static int launch_one(pdes_ctx* c, const double* q, const double* v, double* res
     return launch_cfg<Op, VOL, FLUX, T, HALF, 2>(c, q, v, res, jv, flag, nullptr);
   } else {
     switch (variant) {
-      case 0: return launch_cfg<Op, VOL, FLUX, T, FULL, 2>(c, q, v, res, jv, flag, rk);
-      case 1: return launch_cfg<Op, VOL, FLUX, T, FULL, 3>(c, q, v, res, jv, flag, rk);
-      case 2: return launch_cfg<Op, VOL, FLUX, T, HALF, 2>(c, q, v, res, jv, flag, rk);
-      default: return launch_cfg<Op, VOL, FLUX, T, HALF, 3>(c, q, v, res, jv, flag, rk);
+      case 0: return launch_cfg<Op, VOL, FLUX, T, FULL, 3>(c, q, v, res, jv, flag, rk);
+      case 1: return launch_cfg<Op, VOL, FLUX, T, FULL, 4>(c, q, v, res, jv, flag, rk);
+      case 2: return launch_cfg<Op, VOL, FLUX, T, HALF, 3>(c, q, v, res, jv, flag, rk);
+      default: return launch_cfg<Op, VOL, FLUX, T, HALF, 4>(c, q, v, res, jv, flag, rk);
     }
   }
 }
