@@ -141,20 +141,6 @@ __device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T
   for (int i = 0; i < ND; ++i) flux[i] = sat[i] + ef[i];
 }
 
-// logavg  bc_solvers.jl:942-956.  f = (xi-1)/(xi+1) with xi = aL/aR is
-// formed as (aL-aR)/(aL+aR) (one division instead of two; same value to
-// rounding), and xi is only formed on the log branch.
-template <class T>
-__device__ __forceinline__ T logavg(const T& aL, const T& aR) {
-  const T s = aL + aR;
-  T f = (aL - aR) / s;
-  T u = f * f;
-  T F;
-  if (re(u) < 1e-3) F = 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
-  else F = dlog(aL / aR) / (2.0 * f);
-  return s / (2.0 * F);
-}
-
 // Ismail-Roe parameter vector of one state: z1 = sqrt(rho/p), z_{1+d} = z1 u_d,
 // z_last = sqrt(rho p)  (the per-state part of calcEulerFlux_IR, bc_solvers.jl:776-805)
 template <int DIM, class T>
@@ -167,25 +153,46 @@ __device__ __forceinline__ void ir_zvars(const PhysConst& p, const T* q, T* z) {
   z[DIM + 1] = dsqrt(q[0] * pr);
 }
 
+// The series/log factor of logavg for aL, aR with r = 1/(aL + aR):
+// logavg = (aL + aR) / (2 F) and 1/logavg = 2 F r (bc_solvers.jl:942-956,
+// same branch test on u = f^2).
+template <class T>
+__device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r) {
+  const T f = (aL - aR) * r;
+  const T u = f * f;
+  if (re(u) < 1e-3) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
+  return dlog(aL / aR) / (2.0 * f);
+}
+
 // calcEulerFlux_IR, multi-direction  bc_solvers.jl:776-805 (2D), 842-874 (3D),
-// from precomputed z vectors.  dir: [x][ndir] ; F: [ndir][ndof]
+// from precomputed z vectors.  dir: [x][ndir] ; F: [ndir][ndof].
+// The reference's averages, with its divisions shared: z1 sum s0 = zL0 + zR0
+// (r0 = 1/s0 serves as the reference's 1/(zL1 + zR1) and in logavg(z1)),
+// 1/logavg(z1) = 2 F0 r0, logavg(z4) = s3 / (2 F3), and
+// h_hat = (gamma/(gamma-1)) p2_hat / rho_hat with 1/rho_hat = 2 r0 (2 F3 r3).
 template <int DIM, int NDIR, class T>
 __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const T* zR,
                                           const double* dir, T* F) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
-  T la = logavg(zL[ND - 1], zR[ND - 1]);
-  T zs = zL[0] + zR[0];
-  T rzs = 1.0 / zs;
-  T rho_hat = 0.5 * zs * la;
+  const T s0 = zL[0] + zR[0];
+  const T s3 = zL[ND - 1] + zR[ND - 1];
+  const T r0 = 1.0 / s0;
+  const T r3 = 1.0 / s3;
+  const T F0 = logavg_factor(zL[0], zR[0], r0);
+  const T F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3);
+  const T la = s3 / (2.0 * F3);             // logavg(z4_L, z4_R)
+  const T inv_la0 = 2.0 * F0 * r0;          // 1 / logavg(z1_L, z1_R)
+  const T rho_hat = 0.5 * s0 * la;
   T vh[3];
 #pragma unroll
-  for (int d = 0; d < DIM; ++d) vh[d] = (zL[1 + d] + zR[1 + d]) * rzs;
-  T p1_hat = (zL[ND - 1] + zR[ND - 1]) * rzs;
-  T p2_hat = ((gamma + 1) / (2 * gamma)) * la / logavg(zL[0], zR[0]) + (gamma_1 / (2 * gamma)) * p1_hat;
+  for (int d = 0; d < DIM; ++d) vh[d] = (zL[1 + d] + zR[1 + d]) * r0;
+  const T p1_hat = s3 * r0;
+  const T p2_hat = ((gamma + 1) / (2 * gamma)) * la * inv_la0 + (gamma_1 / (2 * gamma)) * p1_hat;
   T ke = vh[0] * vh[0] + vh[1] * vh[1];
   if constexpr (DIM == 3) ke = ke + vh[2] * vh[2];
-  T h_hat = (gamma / gamma_1) * p2_hat / rho_hat + 0.5 * ke;
+  const T inv_rho_hat = 2.0 * r0 * (2.0 * F3 * r3);
+  const T h_hat = (gamma / gamma_1) * p2_hat * inv_rho_hat + 0.5 * ke;
 #pragma unroll
   for (int i = 0; i < NDIR; ++i) {
     T un = dir[0 * NDIR + i] * vh[0] + dir[1 * NDIR + i] * vh[1];
