@@ -35,16 +35,17 @@ from pdes_amd.batch import BatchSolver  # noqa: E402
 PCS = {"bjacobi": abi.PDES_PC_BJACOBI, "ilu0": abi.PDES_PC_ILU0}
 
 
-def newton_options(pc: str = "ilu0"):
-    return linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=50, pc_kind=PCS[pc])
+def newton_options(pc: str = "ilu0", restart: int = 30):
+    return linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=50, pc_kind=PCS[pc],
+                                  gmres_restart=restart)
 
 
 def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_tol: float = 1e-8,
-        only: str = "both", pc: str = "ilu0"):
+        only: str = "both", pc: str = "ilu0", restart: int = 30):
     t0 = time.perf_counter()
     inst = problems.config3_instances(count=count, seed=seed, nmin=nmin, nmax=nmax)
     t_gen = time.perf_counter() - t0
-    opts = newton_options(pc)
+    opts = newton_options(pc, restart)
     ndof = [int(q.size) for (m, op, ph, q) in inst]
 
     if only == "batched":
@@ -97,6 +98,7 @@ def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_to
         "workload": f"{count} 2D vortex instances (seed {seed}), n in {nmin}..{nmax}, Omega p1-3 Roe / "
                     "diagonal-E p2-3 IR split, Min in [0.3,0.95], Newton to ||R|| <= 1e-10",
         "preconditioner": pc,
+        "gmres_restart": restart,
         "instances": count,
         "total_dofs": int(sum(ndof)),
         "groups": groups,
@@ -127,8 +129,9 @@ def main():
     ap.add_argument("--nmax", type=int, default=128)
     ap.add_argument("--only", choices=["both", "batched"], default="both")
     ap.add_argument("--pc", choices=sorted(PCS), default="ilu0")
+    ap.add_argument("--restart", type=int, default=30)
     a = ap.parse_args()
-    print(json.dumps(run(a.count, a.seed, a.nmin, a.nmax, only=a.only, pc=a.pc)), flush=True)
+    print(json.dumps(run(a.count, a.seed, a.nmin, a.nmax, only=a.only, pc=a.pc, restart=a.restart)), flush=True)
 
 
 if __name__ == "__main__":

@@ -4,6 +4,8 @@
 // cited beside it; the arithmetic sequence is kept so that results agree with
 // the reference to rounding.
 #pragma once
+#include <cstddef>
+#include <type_traits>
 #include "pdes_common.hpp"
 
 namespace pdes {
@@ -156,12 +158,16 @@ __device__ __forceinline__ void ir_zvars(const PhysConst& p, const T* q, T* z) {
 // The series/log factor of logavg for aL, aR with r = 1/(aL + aR):
 // logavg = (aL + aR) / (2 F) and 1/logavg = 2 F r (bc_solvers.jl:942-956,
 // same branch test on u = f^2).
-template <class T>
-__device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r) {
+// When logs of both arguments are at hand (lgL, lgR, e.g. staged per node),
+// log(aL/aR) is taken as lgL - lgR (the branch is only taken for
+// |f| >= 0.0316, where the difference loses no accuracy that matters).
+template <class T, class LP>
+__device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r, LP lgL, LP lgR) {
   const T f = (aL - aR) * r;
   const T u = f * f;
   if (re(u) < 1e-3) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
-  return dlog(aL / aR) / (2.0 * f);
+  if constexpr (std::is_same<LP, std::nullptr_t>::value) return dlog(aL / aR) / (2.0 * f);
+  else return (*lgL - *lgR) / (2.0 * f);
 }
 
 // calcEulerFlux_IR, multi-direction  bc_solvers.jl:776-805 (2D), 842-874 (3D),
@@ -170,17 +176,25 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r)
 // (r0 = 1/s0 serves as the reference's 1/(zL1 + zR1) and in logavg(z1)),
 // 1/logavg(z1) = 2 F0 r0, logavg(z4) = s3 / (2 F3), and
 // h_hat = (gamma/(gamma-1)) p2_hat / rho_hat with 1/rho_hat = 2 r0 (2 F3 r3).
-template <int DIM, int NDIR, class T>
+// lzL / lzR: optional logs {log z1, log z4} of the two states (pointers, read
+// only on the log branch of logavg), or nullptr.
+template <int DIM, int NDIR, class T, class LP = std::nullptr_t>
 __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const T* zR,
-                                          const double* dir, T* F) {
+                                          const double* dir, T* F, LP lzL = nullptr, LP lzR = nullptr) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
   const T s0 = zL[0] + zR[0];
   const T s3 = zL[ND - 1] + zR[ND - 1];
   const T r0 = 1.0 / s0;
   const T r3 = 1.0 / s3;
-  const T F0 = logavg_factor(zL[0], zR[0], r0);
-  const T F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3);
+  T F0, F3;
+  if constexpr (std::is_same<LP, std::nullptr_t>::value) {
+    F0 = logavg_factor(zL[0], zR[0], r0, nullptr, nullptr);
+    F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, nullptr, nullptr);
+  } else {
+    F0 = logavg_factor(zL[0], zR[0], r0, lzL, lzR);
+    F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, lzL + 1, lzR + 1);
+  }
   const T la = s3 / (2.0 * F3);             // logavg(z4_L, z4_R)
   const T inv_la0 = 2.0 * F0 * r0;          // 1 / logavg(z1_L, z1_R)
   const T rho_hat = 0.5 * s0 * la;

@@ -135,6 +135,8 @@ struct ResidualSmem {
   // weak form: volume fluxes; split form: IR z-variables and partner contributions
   T F[EPB][(VOL == VOL_WEAK) ? DIM * NP * ND : NP * ND];
   T C[EPB][(VOL == VOL_SPLIT) ? NP * ND : 1];
+  // split form: per-node logs of the IR variables z1 and z4 (for logavg)
+  T Lz[EPB][(VOL == VOL_SPLIT) ? NP : 1][2];
   T G[EPB][NFACES][NF][ND];
 };
 
@@ -246,6 +248,8 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       ir_zvars<DIM>(phys, qi, z);
 #pragma unroll
       for (int k = 0; k < ND; ++k) sm.F[el][i * ND + k] = z[k];
+      sm.Lz[el][i][0] = dlog(z[0]);
+      sm.Lz[el][i][1] = dlog(z[ND - 1]);
     }
     __syncthreads();
     // circulant distance dd: node i evaluates the pair (i, i+dd mod NP); the
@@ -277,8 +281,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
             for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, jnode)[x * DIM + d];
           T Fd[DIM * ND];
-          if (i_is_j) ir_flux_z<DIM, DIM>(phys, zi, zk, nrmD, Fd);
-          else ir_flux_z<DIM, DIM>(phys, zk, zi, nrmD, Fd);
+          const T* lzi = &sm.Lz[el][i][0];
+          const T* lzk = &sm.Lz[el][kk][0];
+          if (i_is_j) ir_flux_z<DIM, DIM>(phys, zi, zk, nrmD, Fd, lzi, lzk);
+          else ir_flux_z<DIM, DIM>(phys, zk, zi, nrmD, Fd, lzk, lzi);
           // res_a -= 2 S[a][b] F for both members a of the pair
 #pragma unroll
           for (int d = 0; d < DIM; ++d) {
