@@ -36,6 +36,10 @@ extern "C" {
 /* numerical face fluxes (reference FluxDict, src/solver/euler/flux.jl:1229-1242) */
 #define PDES_FLUX_ROE 0               /* RoeFlux    flux.jl:783-795 */
 #define PDES_FLUX_IRSLF 1             /* IRSLFFlux  flux.jl:964-977 */
+/* face_integral_type 2 with ESLFFaceIntegral (faceElementIntegrals.jl:611-630):
+ * entropy-conservative element face integral plus the Lax-Friedrichs entropy
+ * penalty, for dense-face (Omega) operators with volume_integral_type 2 */
+#define PDES_FLUX_ESLF 2
 
 /* Mesh description: the fields of the reference's AbstractDGMesh that the
  * residual reads (docs/src/interfaces.md:191-560).  Layouts (C order):
@@ -177,6 +181,12 @@ int pdes_jacobian_setup(pdes_ctx* ctx, int fp64_blocks);
 int pdes_jacobian_nnzb(pdes_ctx* ctx, int64_t* nrows_blk, int64_t* nnzb);
 int pdes_get_bsr_pattern(pdes_ctx* ctx, int64_t* rowptr, int32_t* colind);
 int pdes_assemble_jacobian(pdes_ctx* ctx);
+/* calcJacobianSparse (src/jacobian/jacobian.jl:365-446): the same BSR values
+ * from directional derivatives of the residual over a distance-2 element
+ * colouring (numColours x Np x ndof derivative evaluations).  Used by
+ * pdes_assemble_jacobian for the entropy-stable element face integral, and as
+ * an independent check of the element-wise Jacobian kernels. */
+int pdes_assemble_jacobian_coloring(pdes_ctx* ctx);
 int pdes_get_bsr_values(pdes_ctx* ctx, void* values_host);
 /* b = J x with the assembled BSR matrix (applyLinearOperator of PetscMatLO,
  * src/linearsolvers/lo_petscmat.jl), host vectors. */

@@ -276,6 +276,21 @@ template <class T> inline void convertToIR(const Params& p, const T* qc, T* qe) 
   }
 }
 
+// convertToConservativeFromIR_ 2D: src/solver/euler/conversion.jl:225-242 ; 3D :244-259
+template <class T> inline void convertToConservativeFromIR(const Params& p, const T* qe, T* qc) {
+  const int dim = p.dim, nd = dim + 2;
+  const double gamma = p.gamma, gamma_1 = p.gamma_1;
+  T k1 = qe[1] * qe[1] + qe[2] * qe[2];
+  if (dim == 3) k1 += qe[3] * qe[3];
+  k1 = 0.5 * gamma_1 * k1 / qe[nd - 1];
+  T s = gamma - gamma_1 * qe[0] + k1;
+  T rho_int = std::exp(-s / gamma_1) * std::pow(gamma_1 / std::pow(-gamma_1 * qe[nd - 1], gamma), 1.0 / gamma_1);
+  rho_int *= gamma_1;
+  qc[0] = -qe[nd - 1] * rho_int;
+  for (int d = 1; d <= dim; ++d) qc[d] = qe[d] * rho_int;
+  qc[nd - 1] = (1.0 - k1) * rho_int / gamma_1;
+}
+
 // getIRA0 2D: src/solver/euler/IR_stab.jl:15-56 ; 3D :58-110.  A0 column-major [ndof][ndof] as A0[col*nd+row]
 template <class T> inline void getIRA0(const Params& p, const T* q, T* A0) {
   const double gamma = p.gamma, gamma_1 = p.gamma_1;

@@ -165,7 +165,74 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
   }
 
   // ---- face integrals
-  {
+  if (ph->face_flux == PDES_FLUX_ESLF) {
+    // face_integral_type 2 with ESLFFaceIntegral on a dense face
+    // (getFaceElementIntegral flux.jl:132-160 -> calcESFaceIntegral
+    // faceElementIntegrals.jl:151-170): calcECFaceIntegral (:58-120) plus
+    // calcEntropyPenaltyIntegral with the LF kernel (:199-290, :455-470)
+    const int st_n = s->stencil;
+    T fluxD[15], wL[15 * 5], wR[15 * 5], wLi[5], wRi[5], qLi[5], qRi[5], qavg[5], dw[5], A0[25], flux[5];
+    double nrmD[9];
+    for (int a = 0; a < dim * dim; ++a) nrmD[a] = 0.0;
+    for (int d = 0; d < dim; ++d) nrmD[d * dim + d] = 1.0;
+    for (int i = 0; i < m->numInterfaces; ++i) {
+      const int* it = m->interfaces + 5 * i;
+      int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
+      const double* nrm_xy = m->nrm_face + (size_t)i * Nf * dim;   // [k][d]
+      // entropy conservative part
+      for (int ii = 0; ii < st_n; ++ii) {
+        int p_i = s->perm[ii * nfaces + fL];
+        const T* qi = q + ((size_t)eL * Np + p_i) * nd;
+        for (int jj = 0; jj < st_n; ++jj) {
+          int p_j = s->perm[jj * nfaces + fR];
+          const T* qj = q + ((size_t)eR * Np + p_j) * nd;
+          calcEulerFlux_IR(p, qi, qj, nrmD, dim, fluxD);
+          for (int d = 0; d < dim; ++d) {
+            double Eij = 0.0;
+            for (int k = 0; k < Nf; ++k) {
+              int kR = s->nbrperm[k * s->norient + o];
+              Eij += s->interp[ii * Nf + k] * s->interp[jj * Nf + kR] * s->wface[k] * nrm_xy[k * dim + d];
+            }
+            for (int pp = 0; pp < nd; ++pp) {
+              acc(res[((size_t)eL * Np + p_i) * nd + pp], T(-Eij * fluxD[d * nd + pp]), absmode);
+              acc(res[((size_t)eR * Np + p_j) * nd + pp], T(Eij * fluxD[d * nd + pp]), absmode);
+            }
+          }
+        }
+      }
+      // entropy penalty: IR variables at the stencil nodes
+      for (int ii = 0; ii < st_n; ++ii) {
+        convertToIR(p, q + ((size_t)eL * Np + s->perm[ii * nfaces + fL]) * nd, &wL[ii * nd]);
+        convertToIR(p, q + ((size_t)eR * Np + s->perm[ii * nfaces + fR]) * nd, &wR[ii * nd]);
+      }
+      for (int k = 0; k < Nf; ++k) {
+        int nk = s->nbrperm[k * s->norient + o];
+        const double* dir = nrm_xy + (size_t)k * dim;
+        for (int c = 0; c < nd; ++c) { wLi[c] = 0.0; wRi[c] = 0.0; }
+        for (int jj = 0; jj < st_n; ++jj) {
+          double iL = s->interp[jj * Nf + k], iR = s->interp[jj * Nf + nk];
+          for (int c = 0; c < nd; ++c) { wLi[c] += iL * wL[jj * nd + c]; wRi[c] += iR * wR[jj * nd + c]; }
+        }
+        convertToConservativeFromIR(p, wLi, qLi);
+        convertToConservativeFromIR(p, wRi, qRi);
+        for (int c = 0; c < nd; ++c) { qavg[c] = 0.5 * (qLi[c] + qRi[c]); dw[c] = wLi[c] - wRi[c]; }
+        getIRA0(p, qavg, A0);
+        T lambda_max = getLambdaMax(p, qavg, dir);
+        for (int r = 0; r < nd; ++r) {
+          T a = 0.0;
+          for (int c = 0; c < nd; ++c) a += A0[c * nd + r] * dw[c];
+          flux[r] = a * lambda_max * s->wface[k];
+        }
+        for (int jj = 0; jj < st_n; ++jj) {
+          int jL = s->perm[jj * nfaces + fL], jR = s->perm[jj * nfaces + fR];
+          for (int c = 0; c < nd; ++c) {
+            acc(res[((size_t)eL * Np + jL) * nd + c], T(-s->interp[jj * Nf + k] * flux[c]), absmode);
+            acc(res[((size_t)eR * Np + jR) * nd + c], T(s->interp[jj * Nf + nk] * flux[c]), absmode);
+          }
+        }
+      }
+    }
+  } else {
     std::vector<T> qL((size_t)Nf * nd), qR((size_t)Nf * nd), ff((size_t)Nf * nd);
     for (int i = 0; i < m->numInterfaces; ++i) {
       const int* it = m->interfaces + 5 * i;

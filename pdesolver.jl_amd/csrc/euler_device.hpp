@@ -238,6 +238,44 @@ __device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* q
   qe[ND - 1] = tmp1;
 }
 
+// LFKernel applyEntropyKernel (faceElementIntegrals.jl:455-470):
+// F += scale * lambda_max(qa, nrm) * A0(qa) dv, with getIRA0 (IR_stab.jl:15-110)
+// applied in factored form: with m the momentum, E the energy,
+// h = (E + p)/rho and s = m . dv_m,
+//   row 0:    rho dv_0 + s + E dv_E
+//   row d:    m_d (dv_0 + s/rho + h dv_E) + p dv_d
+//   row E:    E dv_0 + h s + (rho h^2 - a^2 p/(gamma-1)) dv_E
+// and getLambdaMax (euler_funcs.jl:1887-1910) = |U_n| + |n| a.
+template <int DIM, class T>
+__device__ __forceinline__ void lf_entropy_apply(const PhysConst& p, const T* qa, const T* dv, const double* nrm,
+                                                 double scale, T* F) {
+  constexpr int ND = DIM + 2;
+  const double gamma = p.gamma, gamma_1 = p.gamma_1;
+  T pr = pressure<DIM>(p, qa);
+  T rho = qa[0];
+  T rhoinv = 1.0 / rho;
+  T h = (qa[ND - 1] + pr) * rhoinv;
+  T a2 = gamma * pr * rhoinv;
+  T Un = 0.0;
+  double dA = 0.0;
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    Un = Un + nrm[i] * qa[i + 1] * rhoinv;
+    dA += nrm[i] * nrm[i];
+  }
+  dA = sqrt(dA);
+  const T lambda_max = scale * (absvalue3(Un) + dA * dsqrt(a2));
+  T sm = 0.0;
+#pragma unroll
+  for (int d = 1; d <= DIM; ++d) sm = sm + qa[d] * dv[d];
+  const T dvE = dv[ND - 1];
+  const T common = dv[0] + sm * rhoinv + h * dvE;
+  F[0] = F[0] + lambda_max * (rho * dv[0] + sm + qa[ND - 1] * dvE);
+#pragma unroll
+  for (int d = 1; d <= DIM; ++d) F[d] = F[d] + lambda_max * (qa[d] * common + pr * dv[d]);
+  F[ND - 1] = F[ND - 1] + lambda_max * (qa[ND - 1] * dv[0] + h * sm + (rho * h * h - a2 * pr / gamma_1) * dvE);
+}
+
 // IRSLF face flux: calcEulerFlux_IRSLF bc_solvers.jl:898-909 ->
 // applyEntropyKernel_diagE faceElementIntegrals.jl:510-575 -> LFKernel :455-470
 // with getIRA0 IR_stab.jl:15-110 and getLambdaMax euler_funcs.jl:1887-1910.
@@ -258,36 +296,25 @@ __device__ __forceinline__ void irslf_flux(const PhysConst& p, const T* qL, cons
   T qa[ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i) qa[i] = 0.5 * (qL[i] + qR[i]);
-  // getLambdaMax(params, q_avg, dir) and A0 = dq/dv at q_avg (getIRA0),
-  // applied to dv in factored form: with m the momentum, E the energy,
-  // h = (E + p)/rho and s = m . dv_m,
-  //   row 0:    rho dv_0 + s + E dv_E
-  //   row d:    m_d (dv_0 + s/rho + h dv_E) + p dv_d
-  //   row E:    E dv_0 + h s + (rho h^2 - a^2 p/(gamma-1)) dv_E
+  lf_entropy_apply<DIM>(p, qa, dv, nrm, 1.0, F);
+}
+
+// convertToConservativeFromIR_ (src/solver/euler/conversion.jl:225-259):
+// rho_int = gamma_1 exp(-s/gamma_1) (gamma_1 / (-gamma_1 w_E)^gamma)^(1/gamma_1),
+// taken as gamma_1 exp((log(gamma_1) - gamma log(-gamma_1 w_E) - s) / gamma_1)
+template <int DIM, class T>
+__device__ __forceinline__ void ir_to_conservative(const PhysConst& p, const T* w, T* q) {
+  constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
-  T pr = pressure<DIM>(p, qa);
-  T rho = qa[0];
-  T rhoinv = 1.0 / rho;
-  T h = (qa[ND - 1] + pr) * rhoinv;
-  T a2 = gamma * pr * rhoinv;
-  T Un = 0.0;
-  double dA = 0.0;
+  T k1 = w[1] * w[1] + w[2] * w[2];
+  if constexpr (DIM == 3) k1 = k1 + w[3] * w[3];
+  k1 = 0.5 * gamma_1 * k1 / w[ND - 1];
+  const T s = gamma - gamma_1 * w[0] + k1;
+  const T rho_int = gamma_1 * dexp((log(gamma_1) - gamma * dlog(-gamma_1 * w[ND - 1]) - s) / gamma_1);
+  q[0] = -w[ND - 1] * rho_int;
 #pragma unroll
-  for (int i = 0; i < DIM; ++i) {
-    Un = Un + nrm[i] * qa[i + 1] * rhoinv;
-    dA += nrm[i] * nrm[i];
-  }
-  dA = sqrt(dA);
-  const T lambda_max = absvalue3(Un) + dA * dsqrt(a2);
-  T sm = 0.0;
-#pragma unroll
-  for (int d = 1; d <= DIM; ++d) sm = sm + qa[d] * dv[d];
-  const T dvE = dv[ND - 1];
-  const T common = dv[0] + sm * rhoinv + h * dvE;
-  F[0] = F[0] + lambda_max * (rho * dv[0] + sm + qa[ND - 1] * dvE);
-#pragma unroll
-  for (int d = 1; d <= DIM; ++d) F[d] = F[d] + lambda_max * (qa[d] * common + pr * dv[d]);
-  F[ND - 1] = F[ND - 1] + lambda_max * (qa[ND - 1] * dv[0] + h * sm + (rho * h * h - a2 * pr / gamma_1) * dvE);
+  for (int d = 1; d <= DIM; ++d) q[d] = w[d] * rho_int;
+  q[ND - 1] = (1.0 - k1) * rho_int / gamma_1;
 }
 
 // calcIsentropicVortex  src/solver/euler/common_funcs.jl:25-76 (inflow Mach a parameter)

@@ -12,6 +12,9 @@ int jacobian_free(pdes_ctx* c) {
   if (j->d_colind) hipFree(j->d_colind);
   if (j->d_vals) hipFree(j->d_vals);
   if (j->d_err) hipFree(j->d_err);
+  if (j->d_color_el) hipFree(j->d_color_el);
+  if (j->d_seed) hipFree(j->d_seed);
+  if (j->d_jv) hipFree(j->d_jv);
   delete j;
   c->jac = nullptr;
   return PDES_OK;
@@ -188,8 +191,157 @@ int jacobian_setup(pdes_ctx* c, int fp64) {
   return PDES_OK;
 }
 
+// ---------------------------------------------------------------------------
+// calcJacobianSparse (src/jacobian/jacobian.jl:365-446): with a distance-2
+// colouring of the elements (no two elements of a colour share a neighbour),
+// seeding dof (j, a) of every element of one colour gives, in one derivative
+// evaluation of the residual, column (e, j, a) of the Jacobian in the rows of
+// e and of its neighbours for every element e of that colour.  The derivative
+// is the forward-mode (dual number) residual, the device counterpart of the
+// complex step (applyPerturbation / calcJacCol, jacobian.jl:474-681).
+// ---------------------------------------------------------------------------
+__global__ void seed_kernel(const int32_t* __restrict__ els, int64_t n, int np, int nd, int j, int a, double val,
+                            double* __restrict__ v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[((int64_t)els[t] * np + j) * nd + a] = val;
+}
+
+template <int ND, class BT>
+__global__ void coloring_scatter_kernel(const int32_t* __restrict__ els, int64_t n, int np, int nfaces,
+                                        const ElFace* __restrict__ elface, const int64_t* __restrict__ rowptr,
+                                        const int32_t* __restrict__ colind, BT* __restrict__ vals,
+                                        const double* __restrict__ jv, int j, int a) {
+  // thread -> (element slot, row element r in {e, face neighbours}, row node, row dof)
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per = (int64_t)(nfaces + 1) * np * ND;
+  if (t >= n * per) return;
+  const int64_t s = t / per;
+  int rem = (int)(t - s * per);
+  const int r = rem / (np * ND);
+  rem -= r * np * ND;
+  const int i = rem / ND, rr = rem - i * ND;
+  const int64_t e = els[s];
+  const int64_t re = r == 0 ? e : (int64_t)elface[e * nfaces + r - 1].x;
+  if (re < 0) return;
+  const int64_t row = re * np + i;
+  const int64_t pos = bsr_find(rowptr, colind, row, (int32_t)(e * np + j));
+  if (pos >= 0) vals[pos * ND * ND + rr * ND + a] = (BT)jv[row * ND + rr];
+}
+
+// -M/tau_vec on the diagonal blocks (the implicit-Euler term of applyEuler)
+template <int ND, class BT>
+__global__ void ptc_diag_add_kernel(int64_t nrows, const int64_t* __restrict__ rowptr,
+                                    const int32_t* __restrict__ colind, BT* __restrict__ vals,
+                                    const double* __restrict__ ptc_diag, const double* __restrict__ ptc_scale,
+                                    const int32_t* __restrict__ el_inst, int np) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  const int64_t pos = bsr_find(rowptr, colind, row, (int32_t)row);
+  if (pos < 0) return;
+  const double d = ptc_diag[row] * ptc_scale[el_inst ? el_inst[row / np] : 0];
+#pragma unroll
+  for (int a = 0; a < ND; ++a) vals[pos * ND * ND + a * ND + a] = (BT)((double)vals[pos * ND * ND + a * ND + a] - d);
+}
+
+static int coloring_setup(pdes_ctx* c) {
+  pdes_jacobian* J = c->jac;
+  if (J->d_color_el) return PDES_OK;
+  const int nel = c->numEl, nf = c->nfaces;
+  auto nbr = [&](int e, int f) { return c->h_elface[(size_t)e * nf + f].x; };
+  // greedy distance-2 colouring in element order
+  std::vector<int32_t> color(nel, -1);
+  std::vector<int> mark;
+  int ncolor = 0;
+  for (int e = 0; e < nel; ++e) {
+    mark.assign(ncolor + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+      const int a = nbr(e, f);
+      if (a < 0) continue;
+      if (color[a] >= 0) mark[color[a]] = 1;
+      for (int g = 0; g < nf; ++g) {
+        const int b = nbr(a, g);
+        if (b >= 0 && b != e && color[b] >= 0) mark[color[b]] = 1;
+      }
+    }
+    int col = 0;
+    while (col < ncolor && mark[col]) ++col;
+    color[e] = col;
+    ncolor = std::max(ncolor, col + 1);
+  }
+  J->color_ptr.assign(ncolor + 1, 0);
+  for (int e = 0; e < nel; ++e) J->color_ptr[color[e] + 1]++;
+  for (int k = 0; k < ncolor; ++k) J->color_ptr[k + 1] += J->color_ptr[k];
+  std::vector<int32_t> els(nel);
+  std::vector<int64_t> pos(J->color_ptr.begin(), J->color_ptr.end() - 1);
+  for (int e = 0; e < nel; ++e) els[pos[color[e]]++] = e;
+  if (hipMalloc((void**)&J->d_color_el, std::max(1, nel) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&J->d_seed, c->numDof * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&J->d_jv, c->numDof * sizeof(double)) != hipSuccess) {
+    set_error("device allocation for the coloring Jacobian failed");
+    return PDES_ERR_NOMEM;
+  }
+  PDES_HIP(hipMemcpyAsync(J->d_color_el, els.data(), nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipMemsetAsync(J->d_seed, 0, c->numDof * sizeof(double), c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+template <int ND, class BT>
+static int coloring_run(pdes_ctx* c) {
+  pdes_jacobian* J = c->jac;
+  const int np = c->Np, nf = c->nfaces;
+  // the residual runs over the whole mesh (the work list is not used here)
+  const int32_t nlist_saved = c->nlist;
+  c->nlist = -1;
+  int st = PDES_OK;
+  for (size_t k = 0; k + 1 < J->color_ptr.size() && !st; ++k) {
+    const int64_t n = J->color_ptr[k + 1] - J->color_ptr[k];
+    const int32_t* els = J->d_color_el + J->color_ptr[k];
+    for (int j = 0; j < np && !st; ++j)
+      for (int a = 0; a < ND && !st; ++a) {
+        hipLaunchKernelGGL(seed_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, els, n, np, ND, j,
+                           a, 1.0, J->d_seed);
+        st = launch_residual(c, c->d_q, J->d_seed, nullptr, J->d_jv, false);
+        if (st) break;
+        const int64_t nt = n * (nf + 1) * np * ND;
+        hipLaunchKernelGGL((coloring_scatter_kernel<ND, BT>), dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0,
+                           c->stream, els, n, np, nf, c->d_elface, J->d_rowptr, J->d_colind, (BT*)J->d_vals, J->d_jv,
+                           j, a);
+        hipLaunchKernelGGL(seed_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, els, n, np, ND, j,
+                           a, 0.0, J->d_seed);
+      }
+  }
+  c->nlist = nlist_saved;
+  if (st) return st;
+  if (c->ptc_on) {
+    const int64_t nrows = (int64_t)c->numEl * np;
+    hipLaunchKernelGGL((ptc_diag_add_kernel<ND, BT>), dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, c->stream,
+                       nrows, J->d_rowptr, J->d_colind, (BT*)J->d_vals, c->d_ptc_diag, c->d_ptc_scale, c->d_el_inst,
+                       np);
+  }
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+int assemble_coloring(pdes_ctx* c) {
+  PDES_CHECK(c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  PDES_CHECK(c->nlist < 0, PDES_ERR_UNSUPPORTED, "coloring assembly of a packed batch is not available");
+  int st = coloring_setup(c);
+  if (st) return st;
+  PDES_HIP(hipMemsetAsync(c->jac->d_vals, 0, (size_t)c->jac->nnzb * c->nd * c->nd * (c->jac->fp64 ? 8 : 4),
+                          c->stream));
+  if (c->nd == 4) st = c->jac->fp64 ? coloring_run<4, double>(c) : coloring_run<4, float>(c);
+  else st = c->jac->fp64 ? coloring_run<5, double>(c) : coloring_run<5, float>(c);
+  if (st) return st;
+  c->jac->assembled = true;
+  return PDES_OK;
+}
+
 int assemble(pdes_ctx* c) {
   PDES_CHECK(c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  // no element-wise derivative kernel for the entropy-stable element face
+  // integral: assemble it by colouring
+  if (c->phys.face_flux == PDES_FLUX_ESLF) return assemble_coloring(c);
   int st = launch_assemble(c);
   if (st) return st;
   c->jac->assembled = true;
@@ -205,6 +357,14 @@ extern "C" {
 int pdes_jacobian_setup(pdes_ctx* c, int fp64_blocks) {
   PDES_CHECK(c, PDES_ERR_ARG, "null context");
   return jacobian_setup(c, fp64_blocks);
+}
+
+int pdes_assemble_jacobian_coloring(pdes_ctx* c) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  int st = assemble_coloring(c);
+  if (st) return st;
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
 }
 
 int pdes_jacobian_nnzb(pdes_ctx* c, int64_t* nrows_blk, int64_t* nnzb) {

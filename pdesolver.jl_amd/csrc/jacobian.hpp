@@ -15,11 +15,20 @@ struct pdes_jacobian {
   int32_t* d_err = nullptr;
   std::vector<int64_t> h_rowptr;
   std::vector<int32_t> h_colind;
+  // complex-step coloring assembly (calcJacobianSparse): elements grouped by
+  // a distance-2 colouring, the seed and product vectors
+  std::vector<int64_t> color_ptr;
+  int32_t* d_color_el = nullptr;
+  double* d_seed = nullptr;
+  double* d_jv = nullptr;
 };
 
 namespace pdes {
 int jacobian_setup(pdes_ctx* c, int fp64);
 int assemble(pdes_ctx* c);
+// BSR values from directional derivatives of the residual, one per (colour,
+// node, dof) of a distance-2 element colouring
+int assemble_coloring(pdes_ctx* c);
 // y = alpha J x + beta yadd (yadd may be null)
 int launch_spmv(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta);
 // y -= (M / tau_vec) x for the implicit-Euler globalization (no-op when off)

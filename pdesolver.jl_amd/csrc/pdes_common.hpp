@@ -128,6 +128,14 @@ template <int N> __device__ __forceinline__ dual<N> dpow(const dual<N>& a, doubl
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ double dlog(double a) { return log(a); }
 __device__ __forceinline__ double dpow(double a, double g) { return pow(a, g); }
+__device__ __forceinline__ double dexp(double a) { return exp(a); }
+template <int N> __device__ __forceinline__ dual<N> dexp(const dual<N>& a) {
+  dual<N> r;
+  r.v = exp(a.v);
+#pragma unroll
+  for (int k = 0; k < N; ++k) r.d[k] = r.v * a.d[k];
+  return r;
+}
 
 // absvalue (src/Utils/complexify.jl:25-45): sign flip by the real part
 __device__ __forceinline__ double absvalue(double x) { return fabs(x); }

@@ -137,6 +137,14 @@ struct ResidualSmem {
   T C[EPB][(VOL == VOL_SPLIT) ? NP * ND : 1];
   // split form: per-node logs of the IR variables z1 and z4 (for logavg)
   T Lz[EPB][(VOL == VOL_SPLIT) ? NP : 1][2];
+  // entropy-stable element face integral (dense faces): own IR entropy
+  // variables, the neighbours' stencil z and entropy variables per face, and
+  // the entropy-conservative sums per (face, own stencil node)
+  static constexpr bool ES = (FLUX == PDES_FLUX_ESLF);
+  T W[EPB][ES ? NP : 1][ND];
+  T Zn[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
+  T Wn[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
+  T H[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
   T G[EPB][NFACES][NF][ND];
 };
 
@@ -381,6 +389,148 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = wgt * Fk[c];
       }
     }
+  } else if constexpr (FLUX == PDES_FLUX_ESLF) {
+    // getFaceElementIntegral with ESLFFaceIntegral (flux.jl:132-160,
+    // faceElementIntegrals.jl:58-290, 611-630) on dense faces, owner pull:
+    // every element evaluates the whole integral of each of its faces with
+    // the elementL state first, so both neighbours get bit-identical
+    // contributions.  Stage the IR entropy variables of the own nodes and the
+    // z / entropy variables of the neighbours' face-stencil nodes.
+    if (active) {
+      T w[ND];
+      to_ir_vars<DIM>(phys, qi, w);
+#pragma unroll
+      for (int c = 0; c < ND; ++c) sm.W[el][i][c] = w[c];
+      for (int t = i; t < NFACES * ST; t += NP) {
+        const int f = t / ST, sn = t - f * ST;
+        const ElFace rec = mesh.elface[e * NFACES + f];
+        if (rec.x < 0) continue;
+        const int nodeN = sm.perm[sn][rec.z & 0xff];
+        T qn[ND], zn[ND], wn[ND];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) qn[c] = Io<T>::load(qin, vin, ((int64_t)rec.x * NP + nodeN) * ND + c);
+        ir_zvars<DIM>(phys, qn, zn);
+        to_ir_vars<DIM>(phys, qn, wn);
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+          sm.Zn[el][f][sn][c] = zn[c];
+          sm.Wn[el][f][sn][c] = wn[c];
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      // boundary faces: BC flux at the face nodes; interior faces: the
+      // entropy penalty (calcEntropyPenaltyIntegral :199-290) at face node k
+      for (int slot = i; slot < NFACES * NF; slot += NP) {
+        const int f = slot / NF;
+        const int k = slot - f * NF;
+        const ElFace rec = mesh.elface[e * NFACES + f];
+        T Fk[ND];
+        int m = k;
+        double sign = -1.0;
+        if (rec.x < 0) {
+          T qL[ND], qR[ND];
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qL[c] = 0.0;
+          for (int sn = 0; sn < ST; ++sn) {
+            const double r = sm.interp[sn][k];
+            const int node = sm.perm[sn][f];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) qL[c] = qL[c] + r * sm.q[el][node][c];
+          }
+          double qg[ND];
+          bc_state<DIM>(mesh, rec, k, NF, qg);
+#pragma unroll
+          for (int c = 0; c < ND; ++c) qR[c] = qg[c];
+          roe_flux<DIM>(phys, qL, qR, mesh.nrm_bndry + ((int64_t)rec.y * NF + k) * DIM, Fk);
+#pragma unroll
+          for (int c = 0; c < ND; ++c) Fk[c] = sm.wface[k] * Fk[c];
+        } else {
+          const int o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
+          const int kR = sbp.nbrperm[k * sbp.norient + o];
+          T wl[ND], wr[ND];
+#pragma unroll
+          for (int c = 0; c < ND; ++c) { wl[c] = 0.0; wr[c] = 0.0; }
+          for (int sn = 0; sn < ST; ++sn) {
+            const double rl = sm.interp[sn][k], rr = sm.interp[sn][kR];
+            const int own = sm.perm[sn][f];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+              if (side == 0) {
+                wl[c] = wl[c] + rl * sm.W[el][own][c];
+                wr[c] = wr[c] + rr * sm.Wn[el][f][sn][c];
+              } else {
+                wl[c] = wl[c] + rl * sm.Wn[el][f][sn][c];
+                wr[c] = wr[c] + rr * sm.W[el][own][c];
+              }
+            }
+          }
+          T qa[ND], qlr[ND], dw[ND];
+          ir_to_conservative<DIM>(phys, wl, qa);
+          ir_to_conservative<DIM>(phys, wr, qlr);
+#pragma unroll
+          for (int c = 0; c < ND; ++c) {
+            qa[c] = 0.5 * (qa[c] + qlr[c]);
+            dw[c] = wl[c] - wr[c];
+            Fk[c] = 0.0;
+          }
+          lf_entropy_apply<DIM>(phys, qa, dw, mesh.nrm_face + ((int64_t)rec.y * NF + k) * DIM, sm.wface[k], Fk);
+          if (side == 1) { m = kR; sign = 1.0; }
+        }
+#pragma unroll
+        for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = sign * Fk[c];
+      }
+      // entropy-conservative part (calcECFaceIntegral :58-120): stencil node
+      // t of this element on face f against every stencil node of the
+      // neighbour, E_ij^d = sum_k R[i][k] R[j][kR] B[k] n_d[k], IR flux in
+      // the x, y (, z) directions
+      double ident[DIM * DIM];
+#pragma unroll
+      for (int a = 0; a < DIM * DIM; ++a) ident[a] = (a % (DIM + 1) == 0) ? 1.0 : 0.0;
+      for (int tt = i; tt < NFACES * ST; tt += NP) {
+        const int f = tt / ST, t = tt - f * ST;
+        const ElFace rec = mesh.elface[e * NFACES + f];
+        T h[ND];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) h[c] = 0.0;
+        if (rec.x >= 0) {
+          const int o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
+          const double* nr = mesh.nrm_face + (int64_t)rec.y * NF * DIM;
+          const int own = sm.perm[t][f];
+          T zo[ND];
+#pragma unroll
+          for (int c = 0; c < ND; ++c) zo[c] = sm.F[el][own * ND + c];
+          for (int u = 0; u < ST; ++u) {
+            const int iL = side == 0 ? t : u, jR = side == 0 ? u : t;
+            double E[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) E[d] = 0.0;
+            for (int k = 0; k < NF; ++k) {
+              const int kRk = sbp.nbrperm[k * sbp.norient + o];
+              const double base = sm.interp[iL][k] * sm.interp[jR][kRk] * sm.wface[k];
+#pragma unroll
+              for (int d = 0; d < DIM; ++d) E[d] += base * nr[k * DIM + d];
+            }
+            T zn[ND], Fd[DIM * ND];
+#pragma unroll
+            for (int c = 0; c < ND; ++c) zn[c] = sm.Zn[el][f][u][c];
+            if (side == 0) ir_flux_z<DIM, DIM>(phys, zo, zn, ident, Fd);
+            else ir_flux_z<DIM, DIM>(phys, zn, zo, ident, Fd);
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+#pragma unroll
+              for (int c = 0; c < ND; ++c) h[c] = h[c] + E[d] * Fd[d * ND + c];
+          }
+          if (side == 0) {
+#pragma unroll
+            for (int c = 0; c < ND; ++c) h[c] = -h[c];
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < ND; ++c) sm.H[el][f][t][c] = h[c];
+      }
+    }
   } else
   {
   if (active) {
@@ -496,6 +646,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           const double r = sm.interp[s][mm];
 #pragma unroll
           for (int c = 0; c < ND; ++c) ri[c] = ri[c] + r * sm.G[el][f][mm][c];
+        }
+        if constexpr (FLUX == PDES_FLUX_ESLF) {
+#pragma unroll
+          for (int c = 0; c < ND; ++c) ri[c] = ri[c] + sm.H[el][f][s][c];
         }
       }
     }

@@ -29,7 +29,10 @@ static int launch_cfg(pdes_ctx* c, const double* q, const double* v, double* res
 template <class Op, int VOL, int FLUX, class T>
 static int launch_one(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, int32_t* flag,
                       const RKStage* rk, int variant) {
-  constexpr int FULL = 256 / Op::NP, HALF = 128 / Op::NP;
+  // the entropy-stable face integral stages the neighbours' stencil data in
+  // LDS: half the elements per block
+  constexpr int FULL = (FLUX == PDES_FLUX_ESLF ? 128 : 256) / Op::NP;
+  constexpr int HALF = (FLUX == PDES_FLUX_ESLF ? 64 : 128) / Op::NP;
   if constexpr (!std::is_same<T, double>::value) {
     return launch_cfg<Op, VOL, FLUX, T, HALF, 2>(c, q, v, res, jv, flag, nullptr);
   } else {
@@ -51,6 +54,10 @@ int launch_shape(pdes_ctx* c, const double* q, const double* v, double* res, dou
   if constexpr (Op::DIAGE && Op::DIM == 2) {
     if (vol == 2 && flux == PDES_FLUX_IRSLF)
       return launch_one<Op, VOL_SPLIT, PDES_FLUX_IRSLF, T>(c, q, v, res, jv, flag, rk, variant);
+  }
+  if constexpr (!Op::DIAGE) {
+    if (vol == 2 && flux == PDES_FLUX_ESLF)
+      return launch_one<Op, VOL_SPLIT, PDES_FLUX_ESLF, T>(c, q, v, res, jv, flag, rk, variant);
   }
   set_error("unsupported combination volume_integral_type=%d face_flux=%d for this operator", vol, flux);
   return PDES_ERR_UNSUPPORTED;

@@ -45,6 +45,7 @@ _SIGS = {
     "pdes_jacobian_nnzb": [_vp, _i64p, _i64p],
     "pdes_get_bsr_pattern": [_vp, _vp, _vp],
     "pdes_assemble_jacobian": [_vp],
+    "pdes_assemble_jacobian_coloring": [_vp],
     "pdes_get_bsr_values": [_vp, _vp],
     "pdes_bsr_matvec": [_vp, _vp, _vp],
     "pdes_pc_setup": [_vp, C.c_int],

@@ -20,6 +20,7 @@ PDES_BC_ISENTROPIC_VORTEX = 0
 PDES_BC_FREESTREAM = 1
 PDES_FLUX_ROE = 0
 PDES_FLUX_IRSLF = 1
+PDES_FLUX_ESLF = 2
 PDES_PC_NONE = 0
 PDES_PC_BJACOBI = 1
 PDES_PC_ILU0 = 2

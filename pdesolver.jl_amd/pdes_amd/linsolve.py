@@ -29,6 +29,14 @@ def evalJacobian(mesh, sbp, eqn: EulerData, opts=None, fp64_blocks: bool = False
     check(lib().pdes_assemble_jacobian(eqn.ctx))
 
 
+def calcJacobianSparse(mesh, sbp, eqn: EulerData, opts=None, fp64_blocks: bool = False):
+    """The Jacobian from directional derivatives of the residual over a
+    distance-2 element colouring (calcJacobianSparse,
+    src/jacobian/jacobian.jl:365-446), into the same device BSR matrix."""
+    check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))
+    check(lib().pdes_assemble_jacobian_coloring(eqn.ctx))
+
+
 def getSparsityPattern(eqn: EulerData, fp64_blocks: bool = False):
     """(rowptr, colind) of the node-block BSR matrix (0-based, sorted)."""
     check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))

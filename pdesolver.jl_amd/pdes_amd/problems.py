@@ -14,7 +14,7 @@ import numpy as np
 
 from . import mesh as meshmod
 from . import sbp as sbpmod
-from .abi import Physics, PDES_FLUX_ROE, PDES_FLUX_IRSLF, PDES_BC_FREESTREAM, PDES_BC_ISENTROPIC_VORTEX
+from .abi import Physics, PDES_FLUX_ROE, PDES_FLUX_IRSLF, PDES_FLUX_ESLF, PDES_BC_FREESTREAM, PDES_BC_ISENTROPIC_VORTEX
 
 
 def vortex_exact(coords: np.ndarray, phys: Physics) -> np.ndarray:
@@ -94,7 +94,9 @@ def freestream_bump_state(m: meshmod.DGMesh, phys: Physics, seed: int = 2, amp: 
 def make_problem(kind: str, n: int, p: int, op_kind: str = "omega", flux: str = "roe",
                  Ma: float = 0.5, aoa_deg: float = 0.0, vortex_Min: float = 0.95):
     """Returns (mesh, op, phys).  kind: "vortex" (2D quarter annulus, vortex BC
-    on all boundaries) or "cube" (3D unit cube, freestream BC)."""
+    on all boundaries) or "cube" (3D unit cube, freestream BC).  flux: "roe"
+    (weak form, Roe faces), "ir" (diagonal-E: IR split form, IRSLF faces) or
+    "es" (Omega: IR split form, entropy-stable element face integral)."""
     if kind == "vortex":
         op = sbpmod.get_operator(2, p, op_kind)
         m = meshmod.quarter_annulus(n, op)
@@ -108,6 +110,11 @@ def make_problem(kind: str, n: int, p: int, op_kind: str = "omega", flux: str = 
     if flux == "roe":
         phys = Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=1,
                        face_flux=PDES_FLUX_ROE, bc_type=bc)
+    elif flux == "es":
+        if op.kind != "omega":
+            raise ValueError("the entropy-stable element face integral is for dense-face (Omega) operators")
+        phys = Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=2,
+                       face_flux=PDES_FLUX_ESLF, bc_type=bc)
     elif flux == "ir":
         if op.kind != "diage":
             raise ValueError("the IR split form with IRSLF face flux needs a diagonal-E operator")

@@ -16,6 +16,7 @@ CASES = {
     "vortex_3x3_diage_p2_ir": ("vortex", 3, 2, "diage", "ir"),
     "vortex_2x2_diage_p4_ir": ("vortex", 2, 4, "diage", "ir"),
     "cube_1_omega_p2_roe": ("cube", 1, 2, "omega", "roe"),
+    "vortex_3x3_omega_p2_es": ("vortex", 3, 2, "omega", "es"),
 }
 
 

@@ -16,7 +16,10 @@ CASES = [("vortex", 5, 1, "omega", "roe"), ("vortex", 5, 2, "omega", "roe"),
          ("vortex", 4, 3, "diage", "roe"), ("vortex", 4, 4, "diage", "roe"),
          ("vortex", 5, 1, "diage", "ir"), ("vortex", 5, 2, "diage", "ir"),
          ("vortex", 4, 3, "diage", "ir"), ("vortex", 4, 4, "diage", "ir"),
-         ("cube", 2, 1, "omega", "roe"), ("cube", 2, 2, "omega", "roe")]
+         ("cube", 2, 1, "omega", "roe"), ("cube", 2, 2, "omega", "roe"),
+         ("vortex", 4, 1, "omega", "es"), ("vortex", 4, 2, "omega", "es"),
+         ("vortex", 3, 3, "omega", "es"), ("vortex", 3, 4, "omega", "es"),
+         ("cube", 2, 1, "omega", "es"), ("cube", 2, 2, "omega", "es")]
 
 
 @pytest.mark.parametrize("kind,n,p,opk,flux", CASES)
@@ -37,7 +40,8 @@ def test_residual_parity(oracle, kind, n, p, opk, flux):
     eqn.free()
 
 
-@pytest.mark.parametrize("kind,n,p,opk,flux", [CASES[1], CASES[5], CASES[9], CASES[11], CASES[13]])
+@pytest.mark.parametrize("kind,n,p,opk,flux", [CASES[1], CASES[5], CASES[9], CASES[11], CASES[13], CASES[15],
+                                              CASES[19]])
 def test_jv_matches_complex_step(oracle, kind, n, p, opk, flux):
     """evaldRdqProduct on the device vs the oracle's complex step (h=1e-20)."""
     print(f"case {kind} n={n} p={p} {opk} {flux}", flush=True)

@@ -212,7 +212,9 @@ CASES = [("vortex", 4, 1, "omega", "roe"), ("vortex", 4, 2, "omega", "roe"),
          ("vortex", 4, 3, "omega", "roe"), ("vortex", 3, 4, "omega", "roe"),
          ("vortex", 4, 1, "diage", "roe"), ("vortex", 4, 2, "diage", "roe"),
          ("vortex", 4, 2, "diage", "ir"), ("vortex", 3, 4, "diage", "ir"),
-         ("cube", 2, 1, "omega", "roe"), ("cube", 2, 2, "omega", "roe")]
+         ("cube", 2, 1, "omega", "roe"), ("cube", 2, 2, "omega", "roe"),
+         ("vortex", 4, 1, "omega", "es"), ("vortex", 3, 2, "omega", "es"), ("vortex", 3, 4, "omega", "es"),
+         ("cube", 2, 2, "omega", "es")]
 
 
 @pytest.mark.parametrize("kind,n,p,opk,flux", CASES)
