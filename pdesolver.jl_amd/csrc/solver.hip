@@ -29,6 +29,7 @@ struct pdes_precond {
   std::vector<int64_t> fptr, bptr;
   int32_t* d_fel = nullptr;
   int32_t* d_bel = nullptr;
+  int32_t* d_own = nullptr;   // per node row: offset of the element's own block run within the row
   // the 2 x levels sweep launches of one apply, captured once per (mask,
   // matrix) into a HIP graph over fixed staging vectors and replayed
   double* d_stage_in = nullptr;
@@ -54,6 +55,7 @@ int precond_free(pdes_ctx* c) {
   if (c->pc_obj->d_inv) hipFree(c->pc_obj->d_inv);
   if (c->pc_obj->d_fel) hipFree(c->pc_obj->d_fel);
   if (c->pc_obj->d_bel) hipFree(c->pc_obj->d_bel);
+  if (c->pc_obj->d_own) hipFree(c->pc_obj->d_own);
   if (c->pc_obj->d_stage_in) hipFree(c->pc_obj->d_stage_in);
   if (c->pc_obj->d_stage_out) hipFree(c->pc_obj->d_stage_out);
   for (auto& g : c->pc_obj->graphs) hipGraphExecDestroy(g.exec);
@@ -288,7 +290,8 @@ ilu_factor_level_kernel(const int32_t* __restrict__ lev_el, const int64_t* __res
 template <int NP, int ND, class BT>
 __global__ void __launch_bounds__(256)
 ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const int64_t* __restrict__ rowptr,
-                       const int32_t* __restrict__ colind, const BT* __restrict__ vals, const float* __restrict__ inv,
+                       const int32_t* __restrict__ colind, const int32_t* __restrict__ own,
+                       const BT* __restrict__ vals, const float* __restrict__ inv,
                        const double* __restrict__ b, double* __restrict__ x, int forward,
                        const int32_t* __restrict__ el_inst, const int32_t* __restrict__ mask,
                        const int32_t* __restrict__ inst_active) {
@@ -310,32 +313,21 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
     const int64_t row = e * NP + i;
     double acc = forward ? b[e * N + r] : 0.0;
     // columns are sorted: lower-element blocks precede the element's own run
-    // of NP blocks, upper-element blocks follow it
+    // of NP blocks (starting at own[row] within the row), upper-element
+    // blocks follow it; fixed trip counts so the loads pipeline
     const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
-    const int32_t own0 = (int32_t)(e * NP), own1 = (int32_t)(e * NP + NP);
-    if (forward) {
-      for (int64_t p = p0; p < p1; ++p) {
-        const int32_t col = colind[p];
-        if (col >= own0) break;
-        const BT* blk = vals + p * ND * ND + a * ND;
-        const double* xc = x + (int64_t)col * ND;
-        double t = 0.0;
+    const int64_t q0 = forward ? p0 : p0 + own[row] + NP;
+    const int64_t q1 = forward ? p0 + own[row] : p1;
+    double t = 0.0;
+#pragma unroll 4
+    for (int64_t p = q0; p < q1; ++p) {
+      const int32_t col = colind[p];
+      const BT* blk = vals + p * ND * ND + a * ND;
+      const double* xc = x + (int64_t)col * ND;
 #pragma unroll
-        for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
-        acc -= t;
-      }
-    } else {
-      for (int64_t p = p1 - 1; p >= p0; --p) {
-        const int32_t col = colind[p];
-        if (col < own1) break;
-        const BT* blk = vals + p * ND * ND + a * ND;
-        const double* xc = x + (int64_t)col * ND;
-        double t = 0.0;
-#pragma unroll
-        for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
-        acc += t;
-      }
+      for (int c = 0; c < ND; ++c) t += (double)blk[c] * xc[c];
     }
+    acc = forward ? acc - t : acc + t;
     v[slot][r] = (float)acc;
   }
   __syncthreads();
@@ -382,6 +374,21 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
   std::vector<int32_t> fel, bel;
   bucket(fl, fmax, p->fptr, fel);
   bucket(bl, bmax, p->bptr, bel);
+  // offset of the own-element block run in every row of the BSR pattern
+  const pdes_jacobian* J = c->jac;
+  const int64_t nrows = (int64_t)nel * c->Np;
+  std::vector<int32_t> own(nrows, 0);
+  for (int64_t r = 0; r < nrows; ++r) {
+    const int32_t first = (int32_t)((r / c->Np) * c->Np);
+    int64_t q = J->h_rowptr[r];
+    while (q < J->h_rowptr[r + 1] && J->h_colind[q] < first) ++q;
+    own[r] = (int32_t)(q - J->h_rowptr[r]);
+  }
+  if (hipMalloc((void**)&p->d_own, std::max<int64_t>(1, nrows) * sizeof(int32_t)) != hipSuccess) {
+    set_error("device allocation of the ILU row offsets failed");
+    return PDES_ERR_NOMEM;
+  }
+  PDES_HIP(hipMemcpyAsync(p->d_own, own.data(), nrows * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
   if (hipMalloc((void**)&p->d_fel, std::max(1, nel) * sizeof(int32_t)) != hipSuccess ||
       hipMalloc((void**)&p->d_bel, std::max(1, nel) * sizeof(int32_t)) != hipSuccess) {
     set_error("device allocation of the ILU level lists failed");
@@ -429,11 +436,11 @@ static int ilu_apply(pdes_ctx* c, const double* b, double* x) {
       const dim3 grid((unsigned)ceil_div(cnt, EPB));
       if (j->fp64)
         hipLaunchKernelGGL((ilu_sweep_level_kernel<NP, ND, double>), grid, dim3(256), 0, c->stream, els + ptr[l], cnt,
-                           j->d_rowptr, j->d_colind, (const double*)j->d_vals, p->d_inv, b, x, pass == 0,
+                           j->d_rowptr, j->d_colind, p->d_own, (const double*)j->d_vals, p->d_inv, b, x, pass == 0,
                            c->d_el_inst, c->d_mask, c->d_inst_active);
       else
         hipLaunchKernelGGL((ilu_sweep_level_kernel<NP, ND, float>), grid, dim3(256), 0, c->stream, els + ptr[l], cnt,
-                           j->d_rowptr, j->d_colind, (const float*)j->d_vals, p->d_inv, b, x, pass == 0,
+                           j->d_rowptr, j->d_colind, p->d_own, (const float*)j->d_vals, p->d_inv, b, x, pass == 0,
                            c->d_el_inst, c->d_mask, c->d_inst_active);
     }
   }
