@@ -166,8 +166,8 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r,
   const T f = (aL - aR) * r;
   const T u = f * f;
   if (re(u) < 1e-3) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
-  if constexpr (std::is_same<LP, std::nullptr_t>::value) return dlog(aL / aR) / (2.0 * f);
-  else return (*lgL - *lgR) / (2.0 * f);
+  if constexpr (std::is_same<LP, std::nullptr_t>::value) return dlog(aL / aR) * (0.5 * drcp(f));
+  else return (*lgL - *lgR) * (0.5 * drcp(f));
 }
 
 // calcEulerFlux_IR, multi-direction  bc_solvers.jl:776-805 (2D), 842-874 (3D),
@@ -176,6 +176,7 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r,
 // (r0 = 1/s0 serves as the reference's 1/(zL1 + zR1) and in logavg(z1)),
 // 1/logavg(z1) = 2 F0 r0, logavg(z4) = s3 / (2 F3), and
 // h_hat = (gamma/(gamma-1)) p2_hat / rho_hat with 1/rho_hat = 2 r0 (2 F3 r3).
+// The three reciprocals (r0, r3, 1/F3) are drcp's Newton-refined estimates.
 // lzL / lzR: optional logs {log z1, log z4} of the two states (pointers, read
 // only on the log branch of logavg), or nullptr.
 template <int DIM, int NDIR, class T, class LP = std::nullptr_t>
@@ -185,8 +186,8 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
   const T s0 = zL[0] + zR[0];
   const T s3 = zL[ND - 1] + zR[ND - 1];
-  const T r0 = 1.0 / s0;
-  const T r3 = 1.0 / s3;
+  const T r0 = drcp(s0);
+  const T r3 = drcp(s3);
   T F0, F3;
   if constexpr (std::is_same<LP, std::nullptr_t>::value) {
     F0 = logavg_factor(zL[0], zR[0], r0, nullptr, nullptr);
@@ -195,7 +196,7 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
     F0 = logavg_factor(zL[0], zR[0], r0, lzL, lzR);
     F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, lzL + 1, lzR + 1);
   }
-  const T la = s3 / (2.0 * F3);             // logavg(z4_L, z4_R)
+  const T la = s3 * (0.5 * drcp(F3));       // logavg(z4_L, z4_R)
   const T inv_la0 = 2.0 * F0 * r0;          // 1 / logavg(z1_L, z1_R)
   const T rho_hat = 0.5 * s0 * la;
   T vh[3];

@@ -125,6 +125,26 @@ template <int N> __device__ __forceinline__ dual<N> dpow(const dual<N>& a, doubl
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
   return r;
 }
+// 1/x from the hardware reciprocal estimate (v_rcp_f64) refined by two
+// Newton-Raphson steps r <- r + r (1 - x r), each of which squares the
+// relative error.  Within about one ulp, not correctly rounded; meant for the
+// well-scaled positive sums of the flux functions, far from the exponent
+// limits where an IEEE division needs its scaling steps.
+__device__ __forceinline__ double drcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+template <int N> __device__ __forceinline__ dual<N> drcp(const dual<N>& a) {
+  dual<N> r;
+  r.v = drcp(a.v);
+  const double f = -r.v * r.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
+  return r;
+}
 __device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ double dlog(double a) { return log(a); }
 __device__ __forceinline__ double dpow(double a, double g) { return pow(a, g); }

@@ -288,11 +288,17 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           for (int x = 0; x < DIM; ++x)
 #pragma unroll
             for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, jnode)[x * DIM + d];
-          T Fd[DIM * ND];
-          const T* lzi = &sm.Lz[el][i][0];
-          const T* lzk = &sm.Lz[el][kk][0];
-          if (i_is_j) ir_flux_z<DIM, DIM>(phys, zi, zk, nrmD, Fd, lzi, lzk);
-          else ir_flux_z<DIM, DIM>(phys, zk, zi, nrmD, Fd, lzk, lzi);
+          // operands selected per lane, not branched on: lanes of one wave
+          // hold both orientations, and a branch would run the flux twice
+          T Fd[DIM * ND], zL[ND], zR[ND];
+#pragma unroll
+          for (int k = 0; k < ND; ++k) {
+            zL[k] = i_is_j ? zi[k] : zk[k];
+            zR[k] = i_is_j ? zk[k] : zi[k];
+          }
+          const T* lzL = &sm.Lz[el][jnode][0];
+          const T* lzR = &sm.Lz[el][i_is_j ? kk : i][0];
+          ir_flux_z<DIM, DIM>(phys, zL, zR, nrmD, Fd, lzL, lzR);
           // res_a -= 2 S[a][b] F for both members a of the pair
 #pragma unroll
           for (int d = 0; d < DIM; ++d) {
