@@ -177,9 +177,10 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r,
 // 1/logavg(z1) = 2 F0 r0, logavg(z4) = s3 / (2 F3), and
 // h_hat = (gamma/(gamma-1)) p2_hat / rho_hat with 1/rho_hat = 2 r0 (2 F3 r3).
 // The three reciprocals (r0, r3, 1/F3) are drcp's Newton-refined estimates.
-// lzL / lzR: optional logs {log z1, log z4} of the two states (pointers, read
-// only on the log branch of logavg), or nullptr.
-template <int DIM, int NDIR, class T, class LP = std::nullptr_t>
+// lzL / lzR: optional logs {log z1, log z4} of the two states (pointers to
+// log z1, with log z4 LS elements further; read only on the log branch of
+// logavg), or nullptr.
+template <int DIM, int NDIR, class T, class LP = std::nullptr_t, int LS = 1>
 __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const T* zR,
                                           const double* dir, T* F, LP lzL = nullptr, LP lzR = nullptr) {
   constexpr int ND = DIM + 2;
@@ -194,7 +195,7 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
     F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, nullptr, nullptr);
   } else {
     F0 = logavg_factor(zL[0], zR[0], r0, lzL, lzR);
-    F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, lzL + 1, lzR + 1);
+    F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, lzL + LS, lzR + LS);
   }
   const T la = s3 * (0.5 * drcp(F3));       // logavg(z4_L, z4_R)
   const T inv_la0 = 2.0 * F0 * r0;          // 1 / logavg(z1_L, z1_R)

@@ -133,8 +133,12 @@ struct ResidualSmem {
   double wface[NF];
   T q[EPB][NP][ND];
   // weak form: volume fluxes; split form: IR z-variables and partner contributions
+  // (the split form's arrays are planes of one value per (element, node),
+  //  [component][element][node], so that the lanes of a pair round, which
+  //  read node i + dd, touch consecutive words)
   T F[EPB][(VOL == VOL_WEAK) ? DIM * NP * ND : NP * ND];
-  T C[EPB][(VOL == VOL_SPLIT) ? NP * ND : 1];
+  // split form: partner contributions, two buffers used in alternate rounds
+  T C[(VOL == VOL_SPLIT) ? 2 : 1][EPB][(VOL == VOL_SPLIT) ? NP * ND : 1];
   // split form: per-node logs of the IR variables z1 and z4 (for logavg)
   T Lz[EPB][(VOL == VOL_SPLIT) ? NP : 1][2];
   // entropy-stable element face integral (dense faces): own IR entropy
@@ -160,7 +164,20 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 
   const int tid = threadIdx.x;
   // stage operator tables
-  for (int x = tid; x < DIM * NP * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.Q[x];
+  if constexpr (VOL == VOL_SPLIT) {
+    // the split form only needs the antisymmetric part Q - Q^T = 2 S, and
+    // only in the order the pair rounds read it: entry [d][dd][a] holds
+    // 2 S_d[a][a+dd mod NP], so the lanes of one round read consecutive words
+    constexpr int NS = NP / 2 + 1;
+    for (int x = tid; x < DIM * NS * NP; x += blockDim.x) {
+      const int d = x / (NS * NP), dd = (x / NP) % NS, a = x % NP;
+      int b = a + dd;
+      if (b >= NP) b -= NP;
+      (&sm.Q[0][0][0])[x] = sbp.Q[(d * NP + a) * NP + b] - sbp.Q[(d * NP + b) * NP + a];
+    }
+  } else {
+    for (int x = tid; x < DIM * NP * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.Q[x];
+  }
   for (int x = tid; x < ST * NF; x += blockDim.x) (&sm.interp[0][0])[x] = sbp.interp[x];
   for (int x = tid; x < ST * NFACES; x += blockDim.x) (&sm.perm[0][0])[x] = sbp.perm[x];
   for (int x = tid; x < NP * NFACES; x += blockDim.x) (&sm.nodepos[0][0])[x] = sbp.nodepos[x];
@@ -251,76 +268,88 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   } else {
     // split form (calcVolumeIntegralsSplitFormLinear): per-node IR z-variables in LDS,
     // each unordered node pair evaluated once by the node that starts its circulant distance.
+    // LDS planes: value c of node n of this block's element el at [c * PL + el * NP + n]
+    constexpr int PL = EPB * NP;
+    T* const zpl = &sm.F[0][0];
+    T* const lpl = &sm.Lz[0][0][0];
+    const int me = el * NP;
     if (active) {
       T z[ND];
       ir_zvars<DIM>(phys, qi, z);
 #pragma unroll
-      for (int k = 0; k < ND; ++k) sm.F[el][i * ND + k] = z[k];
-      sm.Lz[el][i][0] = dlog(z[0]);
-      sm.Lz[el][i][1] = dlog(z[ND - 1]);
+      for (int k = 0; k < ND; ++k) zpl[k * PL + me + i] = z[k];
+      lpl[me + i] = dlog(z[0]);
+      lpl[PL + me + i] = dlog(z[ND - 1]);
     }
     __syncthreads();
     // circulant distance dd: node i evaluates the pair (i, i+dd mod NP); the
     // partner's share goes through one LDS slot per node, one round per dd.
+    // The slots alternate between two buffers, so one barrier per round
+    // suffices: a thread writes a buffer again only after the next round's
+    // barrier, which every thread passes after its reads of that buffer.
     constexpr int NPAIR = NP / 2;
-    T zi[ND];
-    if (active) {
-#pragma unroll
-      for (int k = 0; k < ND; ++k) zi[k] = sm.F[el][i * ND + k];
-    }
+    const double* const s2pl = &sm.Q[0][0][0];   // 2 S by circulant distance, [d][dd][i]
     for (int dd = 1; dd <= NPAIR; ++dd) {
       const bool half = (2 * dd == NP);            // even NP: distance NP/2 owned by i < NP/2
+      T* const cbuf = &sm.C[dd & 1][0][0];
       if (active) {
-        T cother[ND];
+        T g[ND];
 #pragma unroll
-        for (int k = 0; k < ND; ++k) cother[k] = 0.0;
+        for (int k = 0; k < ND; ++k) g[k] = 0.0;
         if (!(half && i >= NP / 2)) {
           int kk = i + dd;
           if (kk >= NP) kk -= NP;
-          T zk[ND];
-#pragma unroll
-          for (int k = 0; k < ND; ++k) zk[k] = sm.F[el][kk * ND + k];
-          // reference orientation: F(q_j, q_k) with j > k, direction rows of dxidx at node j
+          // reference orientation: F(q_j, q_k) with j > k, direction rows of
+          // dxidx at node j.  The states are read from LDS in that order
+          // (lanes of one wave hold both orientations; no branch, no select).
           const bool i_is_j = i > kk;
           const int jnode = i_is_j ? i : kk;
+          const int knode = i_is_j ? kk : i;
           double nrmD[DIM * DIM];   // [x][d]
+          if (mesh.metric_per_node && !i_is_j) {
 #pragma unroll
-          for (int x = 0; x < DIM; ++x)
+            for (int x = 0; x < DIM; ++x)
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, jnode)[x * DIM + d];
-          // operands selected per lane, not branched on: lanes of one wave
-          // hold both orientations, and a branch would run the flux twice
-          T Fd[DIM * ND], zL[ND], zR[ND];
+              for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, kk)[x * DIM + d];
+          } else {
+            // node j is this node, or the metric is the element's constant one
 #pragma unroll
-          for (int k = 0; k < ND; ++k) {
-            zL[k] = i_is_j ? zi[k] : zk[k];
-            zR[k] = i_is_j ? zk[k] : zi[k];
+            for (int x = 0; x < DIM; ++x)
+#pragma unroll
+              for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = dxi[x][d];
           }
-          const T* lzL = &sm.Lz[el][jnode][0];
-          const T* lzR = &sm.Lz[el][i_is_j ? kk : i][0];
-          ir_flux_z<DIM, DIM>(phys, zL, zR, nrmD, Fd, lzL, lzR);
-          // res_a -= 2 S[a][b] F for both members a of the pair
+          // sum_d 2 S_d[i][kk] F_d is the IR flux in the one direction
+          // n = sum_d 2 S_d[i][kk] dxidx[d, :] (the flux is linear in the direction)
+          double n[DIM];
+#pragma unroll
+          for (int x = 0; x < DIM; ++x) n[x] = 0.0;
 #pragma unroll
           for (int d = 0; d < DIM; ++d) {
-            const double s_ik = 0.5 * (sm.Q[d][i][kk] - sm.Q[d][kk][i]);
+            const double s2 = s2pl[(d * (NPAIR + 1) + dd) * NP + i];
 #pragma unroll
-            for (int k = 0; k < ND; ++k) {
-              ri[k] = ri[k] - 2 * s_ik * Fd[d * ND + k];
-              cother[k] = cother[k] + 2 * s_ik * Fd[d * ND + k];   // -2 S[kk][i] F
-            }
+            for (int x = 0; x < DIM; ++x) n[x] = n[x] + s2 * nrmD[x * DIM + d];
           }
+          T zL[ND], zR[ND];
+#pragma unroll
+          for (int k = 0; k < ND; ++k) {
+            zL[k] = zpl[k * PL + me + jnode];
+            zR[k] = zpl[k * PL + me + knode];
+          }
+          ir_flux_z<DIM, 1, T, const T*, PL>(phys, zL, zR, n, g, lpl + me + jnode, lpl + me + knode);
+          // res_i -= 2 S[i][kk] F, and the partner gets -2 S[kk][i] F = +g
+#pragma unroll
+          for (int k = 0; k < ND; ++k) ri[k] = ri[k] - g[k];
         }
 #pragma unroll
-        for (int k = 0; k < ND; ++k) sm.C[el][i * ND + k] = cother[k];
+        for (int k = 0; k < ND; ++k) cbuf[k * PL + me + i] = g[k];
       }
       __syncthreads();
       if (active) {
         int src = i - dd;
         if (src < 0) src += NP;
 #pragma unroll
-        for (int k = 0; k < ND; ++k) ri[k] = ri[k] + sm.C[el][src * ND + k];
+        for (int k = 0; k < ND; ++k) ri[k] = ri[k] + cbuf[k * PL + me + src];
       }
-      __syncthreads();
     }
   }
 
@@ -506,7 +535,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           const int own = sm.perm[t][f];
           T zo[ND];
 #pragma unroll
-          for (int c = 0; c < ND; ++c) zo[c] = sm.F[el][own * ND + c];
+          for (int c = 0; c < ND; ++c) zo[c] = (&sm.F[0][0])[c * (EPB * NP) + el * NP + own];
           for (int u = 0; u < ST; ++u) {
             const int iL = side == 0 ? t : u, jR = side == 0 ? u : t;
             double E[DIM];
@@ -518,11 +547,14 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
               for (int d = 0; d < DIM; ++d) E[d] += base * nr[k * DIM + d];
             }
-            T zn[ND], Fd[DIM * ND];
+            T zL[ND], zR[ND], Fd[DIM * ND];
 #pragma unroll
-            for (int c = 0; c < ND; ++c) zn[c] = sm.Zn[el][f][u][c];
-            if (side == 0) ir_flux_z<DIM, DIM>(phys, zo, zn, ident, Fd);
-            else ir_flux_z<DIM, DIM>(phys, zn, zo, ident, Fd);
+            for (int c = 0; c < ND; ++c) {
+              const T zn = sm.Zn[el][f][u][c];
+              zL[c] = side == 0 ? zo[c] : zn;
+              zR[c] = side == 0 ? zn : zo[c];
+            }
+            ir_flux_z<DIM, DIM>(phys, zL, zR, ident, Fd);
 #pragma unroll
             for (int d = 0; d < DIM; ++d)
 #pragma unroll

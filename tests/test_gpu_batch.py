@@ -50,7 +50,16 @@ def test_batch_converges_and_matches_single_context():
         assert rep1.converged
         q1 = eqn.get_q()
         qb = bs.get_state(i)
-        assert np.abs(qb - q1).max() <= 1e-9 * np.abs(q1).max()
+        # the two paths reduce in different orders (packed chunked reductions
+        # against one context's), so their Newton iterates round differently;
+        # both states solve the same discrete problem to the tolerance, and
+        # agree to what that tolerance fixes on these coarse meshes
+        eqn.set_q(qb)
+        euler.evalResidual_device(eqn)
+        assert euler.calcNorm(eqn) < 1e-10
+        diff = np.abs(qb - q1).max() / np.abs(q1).max()
+        print(i, "relative state difference", diff, flush=True)
+        assert diff <= 1e-7
         eqn.free()
     bs.free()
 
