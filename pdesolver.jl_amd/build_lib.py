@@ -14,7 +14,10 @@ OUT = os.environ.get("PDES_BUILD_OUT", os.path.join(HERE, "pdes_amd", "libpdes_a
 ARCH = os.environ.get("PDES_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# -ffp-contract=on: a multiply is fused with an add only inside one source
+# expression, so the bits of a result do not depend on how the optimizer laid
+# out a kernel (the launch variants of one kernel give identical results).
+FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-parameter"] + os.environ.get("PDES_EXTRA_FLAGS", "").split()
 

@@ -221,9 +221,12 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
   }
 }
 
-// convertToIR_  src/solver/euler/conversion.jl:160-205
-template <int DIM, class T>
-__device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* qe) {
+// convertToIR_  src/solver/euler/conversion.jl:160-205.  With LZ set, the
+// entropy s = log(p) - gamma log(rho) comes from the logs of the IR parameter
+// vector, log rho = log z1 + log z4 and log p = log z4 - log z1 (lz1 and
+// lz4 given), instead of two logs of the state.
+template <int DIM, class T, bool LZ = false>
+__device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* qe, T lz1 = 0.0, T lz4 = 0.0) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1, gamma_1i = 1.0 / gamma_1;
   T k1 = qc[1] * qc[1] + qc[2] * qc[2];
@@ -231,7 +234,9 @@ __device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* q
   k1 = 0.5 * k1 / qc[0];
   T rho_int = qc[ND - 1] - k1;
   // log(gamma_1 rho_int / rho^gamma) taken as log(gamma_1 rho_int) - gamma log(rho)
-  T s = dlog(gamma_1 * rho_int) - gamma * dlog(qc[0]);
+  T s;
+  if constexpr (LZ) s = (lz4 - lz1) - gamma * (lz1 + lz4);
+  else s = dlog(gamma_1 * rho_int) - gamma * dlog(qc[0]);
   T fac = 1.0 / rho_int;
   T tmp1 = -qc[0] * fac * gamma_1i;
   qe[0] = ((rho_int * (gamma + 1.0 - s) - qc[ND - 1]) * fac) * gamma_1i;
@@ -293,6 +298,25 @@ __device__ __forceinline__ void irslf_flux(const PhysConst& p, const T* qL, cons
   T dv[ND], vR[ND];
   to_ir_vars<DIM>(p, qL, dv);
   to_ir_vars<DIM>(p, qR, vR);
+#pragma unroll
+  for (int i = 0; i < ND; ++i) dv[i] = dv[i] - vR[i];
+  T qa[ND];
+#pragma unroll
+  for (int i = 0; i < ND; ++i) qa[i] = 0.5 * (qL[i] + qR[i]);
+  lf_entropy_apply<DIM>(p, qa, dv, nrm, 1.0, F);
+}
+
+// irslf_flux from the IR parameter vectors of both states and their logs
+// (lzL, lzR point to log z1, with log z4 LS elements further), for kernels
+// that hold them per node already: no square roots or logs of the own state.
+template <int DIM, class T, int LS = 1>
+__device__ __forceinline__ void irslf_flux_z(const PhysConst& p, const T* qL, const T* qR, const T* zL,
+                                             const T* zR, const T* lzL, const T* lzR, const double* nrm, T* F) {
+  constexpr int ND = DIM + 2;
+  ir_flux_z<DIM, 1, T, const T*, LS>(p, zL, zR, nrm, F, lzL, lzR);
+  T dv[ND], vR[ND];
+  to_ir_vars<DIM, T, true>(p, qL, dv, lzL[0], lzL[LS]);
+  to_ir_vars<DIM, T, true>(p, qR, vR, lzR[0], lzR[LS]);
 #pragma unroll
   for (int i = 0; i < ND; ++i) dv[i] = dv[i] - vR[i];
   T qa[ND];

@@ -185,6 +185,12 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 
   const int el = tid / NP;
   const int i = tid - el * NP;
+  // split form: LDS planes, value c of node n of this block's element el at
+  // [c * PL + el * NP + n] (IR parameter vector z in zpl, log z1 / log z4 in lpl)
+  constexpr int PL = EPB * NP;
+  T* const zpl = &sm.F[0][0];
+  T* const lpl = &sm.Lz[0][0][0];
+  const int me = el * NP;
   const int64_t eslot = (int64_t)blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (eslot < launch_count(mesh));
   const int64_t e = active ? element_of(mesh, eslot) : 0;
@@ -268,11 +274,6 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   } else {
     // split form (calcVolumeIntegralsSplitFormLinear): per-node IR z-variables in LDS,
     // each unordered node pair evaluated once by the node that starts its circulant distance.
-    // LDS planes: value c of node n of this block's element el at [c * PL + el * NP + n]
-    constexpr int PL = EPB * NP;
-    T* const zpl = &sm.F[0][0];
-    T* const lpl = &sm.Lz[0][0][0];
-    const int me = el * NP;
     if (active) {
       T z[ND];
       ir_zvars<DIM>(phys, qi, z);
@@ -403,21 +404,41 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         } else {
           const int o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
           const int kR = sbp.nbrperm[k * sbp.norient + o];
-          if (side == 0) {
-            m = k;
-            sign = -1.0;
-            const int node = sm.perm[k][f];
+          m = side == 0 ? k : kR;
+          sign = side == 0 ? -1.0 : 1.0;
+          const int node = sm.perm[m][f];
 #pragma unroll
-            for (int c = 0; c < ND; ++c) { qL[c] = sm.q[el][node][c]; qR[c] = qn[c]; }
-          } else {
-            m = kR;
-            sign = 1.0;
-            const int node = sm.perm[kR][f];
-#pragma unroll
-            for (int c = 0; c < ND; ++c) { qL[c] = qn[c]; qR[c] = sm.q[el][node][c]; }
+          for (int c = 0; c < ND; ++c) {
+            const T qo = sm.q[el][node][c];
+            qL[c] = side == 0 ? qo : qn[c];
+            qR[c] = side == 0 ? qn[c] : qo;
           }
-          if constexpr (FLUX == PDES_FLUX_ROE) roe_flux<DIM>(phys, qL, qR, nrm, Fk);
-          else irslf_flux<DIM>(phys, qL, qR, nrm, Fk);
+          if constexpr (FLUX == PDES_FLUX_ROE) {
+            roe_flux<DIM>(phys, qL, qR, nrm, Fk);
+          } else if constexpr (VOL == VOL_SPLIT) {
+            // the own node's z and logs are in the LDS planes already; the
+            // neighbour's are computed here by the same functions, so the
+            // two elements of the face evaluate the same bits
+            T zn[ND], zL[ND], zR[ND], ln[2], lL[2], lR[2];
+            ir_zvars<DIM>(phys, qn, zn);
+            ln[0] = dlog(zn[0]);
+            ln[1] = dlog(zn[ND - 1]);
+#pragma unroll
+            for (int c = 0; c < ND; ++c) {
+              const T zo = zpl[c * PL + me + node];
+              zL[c] = side == 0 ? zo : zn[c];
+              zR[c] = side == 0 ? zn[c] : zo;
+            }
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+              const T lo = lpl[c * PL + me + node];
+              lL[c] = side == 0 ? lo : ln[c];
+              lR[c] = side == 0 ? ln[c] : lo;
+            }
+            irslf_flux_z<DIM>(phys, qL, qR, zL, zR, lL, lR, nrm, Fk);
+          } else {
+            irslf_flux<DIM>(phys, qL, qR, nrm, Fk);
+          }
         }
         const double wgt = sign * sm.wface[m];
 #pragma unroll
