@@ -51,6 +51,38 @@ struct dual {
 template <int N> __device__ __forceinline__ double re(const dual<N>& a) { return a.v; }
 __device__ __forceinline__ double re(double a) { return a; }
 
+// 1/x from the hardware reciprocal estimate (v_rcp_f64) refined by two
+// Newton-Raphson steps r <- r + r (1 - x r), each of which squares the
+// relative error.  Within about one ulp, not correctly rounded; meant for the
+// well-scaled positive sums of the flux functions, far from the exponent
+// limits where an IEEE division needs its scaling steps.
+__device__ __forceinline__ double drcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+// sqrt(x) for positive normal x: the hardware reciprocal square root estimate
+// (v_rsq_f64) refined by the coupled Newton iteration for g ~ sqrt(x) and
+// h ~ 1/(2 sqrt(x)) (Goldschmidt's square root iteration, as in P. Markstein,
+// "IA-64 and Elementary Functions", 2000), ending with two corrections
+// g += (x - g^2) h.  Within about one ulp.  Zero, subnormal, infinite and
+// negative arguments are not handled: the flux functions take square roots of
+// positive physical quantities only, and non-physical states are flagged by
+// the positivity checks.
+__device__ __forceinline__ double dsqrt(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  const double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  double d = fma(-g, g, x);
+  g = fma(d, h, g);
+  d = fma(-g, g, x);
+  return fma(d, h, g);
+}
+
 #define DUAL_BINOP(OP, VEXPR, DEXPR)                                              \
   template <int N> __device__ __forceinline__ dual<N> operator OP(const dual<N>& a, const dual<N>& b) { \
     dual<N> r; r.v = VEXPR;                                                       \
@@ -108,7 +140,7 @@ template <int N> __device__ __forceinline__ dual<N>& operator+=(dual<N>& a, cons
 template <int N> __device__ __forceinline__ dual<N>& operator-=(dual<N>& a, const dual<N>& b) { a = a - b; return a; }
 
 template <int N> __device__ __forceinline__ dual<N> dsqrt(const dual<N>& a) {
-  dual<N> r; r.v = sqrt(a.v); double f = 0.5 / r.v;
+  dual<N> r; r.v = dsqrt(a.v); double f = 0.5 * drcp(r.v);
 #pragma unroll
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
   return r;
@@ -125,18 +157,6 @@ template <int N> __device__ __forceinline__ dual<N> dpow(const dual<N>& a, doubl
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
   return r;
 }
-// 1/x from the hardware reciprocal estimate (v_rcp_f64) refined by two
-// Newton-Raphson steps r <- r + r (1 - x r), each of which squares the
-// relative error.  Within about one ulp, not correctly rounded; meant for the
-// well-scaled positive sums of the flux functions, far from the exponent
-// limits where an IEEE division needs its scaling steps.
-__device__ __forceinline__ double drcp(double x) {
-  double r = __builtin_amdgcn_rcp(x);
-  double e = fma(-x, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-x, r, 1.0);
-  return fma(r, e, r);
-}
 template <int N> __device__ __forceinline__ dual<N> drcp(const dual<N>& a) {
   dual<N> r;
   r.v = drcp(a.v);
@@ -145,7 +165,6 @@ template <int N> __device__ __forceinline__ dual<N> drcp(const dual<N>& a) {
   for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
   return r;
 }
-__device__ __forceinline__ double dsqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ double dlog(double a) { return log(a); }
 __device__ __forceinline__ double dpow(double a, double g) { return pow(a, g); }
 __device__ __forceinline__ double dexp(double a) { return exp(a); }
