@@ -133,7 +133,7 @@ struct ResidualSmem {
   double wface[NF];
   T q[EPB][NP][ND];
   // weak form: volume fluxes; split form: IR z-variables and partner contributions
-  // (the split form's arrays are planes of one value per (element, node),
+  // (the F, C and Lz arrays are planes of one value per (element, node),
   //  [component][element][node], so that the lanes of a pair round, which
   //  read node i + dd, touch consecutive words)
   T F[EPB][(VOL == VOL_WEAK) ? DIM * NP * ND : NP * ND];
@@ -185,12 +185,16 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 
   const int el = tid / NP;
   const int i = tid - el * NP;
-  // split form: LDS planes, value c of node n of this block's element el at
-  // [c * PL + el * NP + n] (IR parameter vector z in zpl, log z1 / log z4 in lpl)
+  // LDS planes: value c of node n of this block's element el at
+  // [c * PL + el * NP + n] (weak form: component k of the flux in parametric
+  // direction d as c = d * ND + k; split form: IR parameter vector z in zpl,
+  // log z1 / log z4 in lpl)
   constexpr int PL = EPB * NP;
   T* const zpl = &sm.F[0][0];
   T* const lpl = &sm.Lz[0][0][0];
   const int me = el * NP;
+  // face contributions, planes [c][element][face][face node]
+  T* const gpl = &sm.G[0][0][0][0];
   const int64_t eslot = (int64_t)blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (eslot < launch_count(mesh));
   const int64_t e = active ? element_of(mesh, eslot) : 0;
@@ -256,7 +260,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         T Fd[ND];
         euler_flux<DIM>(phys, qi, nrm, Fd);
 #pragma unroll
-        for (int k = 0; k < ND; ++k) sm.F[el][(d * NP + i) * ND + k] = Fd[k];
+        for (int k = 0; k < ND; ++k) zpl[(d * ND + k) * PL + me + i] = Fd[k];
       }
     }
     __syncthreads();
@@ -268,7 +272,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         for (int j = 0; j < NP; ++j) {
           double qji = sm.Q[d][j][i];
 #pragma unroll
-          for (int k = 0; k < ND; ++k) ri[k] = ri[k] + qji * sm.F[el][(d * NP + j) * ND + k];
+          for (int k = 0; k < ND; ++k) ri[k] = ri[k] + qji * zpl[(d * ND + k) * PL + me + j];
         }
     }
   } else {
@@ -442,7 +446,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         }
         const double wgt = sign * sm.wface[m];
 #pragma unroll
-        for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = wgt * Fk[c];
+        for (int c = 0; c < ND; ++c) gpl[((c * EPB + el) * NFACES + f) * NF + m] = wgt * Fk[c];
       }
     }
   } else if constexpr (FLUX == PDES_FLUX_ESLF) {
@@ -535,7 +539,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           if (side == 1) { m = kR; sign = 1.0; }
         }
 #pragma unroll
-        for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = sign * Fk[c];
+        for (int c = 0; c < ND; ++c) gpl[((c * EPB + el) * NFACES + f) * NF + m] = sign * Fk[c];
       }
       // entropy-conservative part (calcECFaceIntegral :58-120): stencil node
       // t of this element on face f against every stencil node of the
@@ -685,7 +689,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       }
       const double wgt = sign * sm.wface[m];
 #pragma unroll
-      for (int c = 0; c < ND; ++c) sm.G[el][f][m][c] = wgt * Fk[c];
+      for (int c = 0; c < ND; ++c) gpl[((c * EPB + el) * NFACES + f) * NF + m] = wgt * Fk[c];
     }
   }
   }
@@ -698,13 +702,13 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       if (s < 0) continue;
       if constexpr (Op::DIAGE) {
 #pragma unroll
-        for (int c = 0; c < ND; ++c) ri[c] = ri[c] + sm.G[el][f][s][c];
+        for (int c = 0; c < ND; ++c) ri[c] = ri[c] + gpl[((c * EPB + el) * NFACES + f) * NF + s];
       } else {
 #pragma unroll
         for (int mm = 0; mm < NF; ++mm) {
           const double r = sm.interp[s][mm];
 #pragma unroll
-          for (int c = 0; c < ND; ++c) ri[c] = ri[c] + r * sm.G[el][f][mm][c];
+          for (int c = 0; c < ND; ++c) ri[c] = ri[c] + r * gpl[((c * EPB + el) * NFACES + f) * NF + mm];
         }
         if constexpr (FLUX == PDES_FLUX_ESLF) {
 #pragma unroll
