@@ -149,7 +149,11 @@ struct ResidualSmem {
   T Zn[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
   T Wn[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
   T H[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
-  T G[EPB][NFACES][NF][ND];
+  // face contributions; the weak form keeps them in the flux planes F, which
+  // are dead once the volume term is summed (one more barrier, less LDS)
+  static constexpr bool G_IN_F = (VOL == VOL_WEAK);
+  static_assert(!G_IN_F || DIM * NP >= NFACES * NF, "face contributions must fit the weak-form flux planes");
+  T G[G_IN_F ? 1 : EPB][NFACES][NF][ND];
 };
 
 template <class Op, int VOL, int FLUX, int EPB, class T, bool RK = false, int WAVES = 2>
@@ -194,7 +198,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   T* const lpl = &sm.Lz[0][0][0];
   const int me = el * NP;
   // face contributions, planes [c][element][face][face node]
-  T* const gpl = &sm.G[0][0][0][0];
+  T* const gpl = Smem::G_IN_F ? &sm.F[0][0] : &sm.G[0][0][0][0];
   const int64_t eslot = (int64_t)blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (eslot < launch_count(mesh));
   const int64_t e = active ? element_of(mesh, eslot) : 0;
@@ -275,6 +279,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           for (int k = 0; k < ND; ++k) ri[k] = ri[k] + qji * zpl[(d * ND + k) * PL + me + j];
         }
     }
+    __syncthreads();   // the face phase writes its contributions into the flux planes
   } else {
     // split form (calcVolumeIntegralsSplitFormLinear): per-node IR z-variables in LDS,
     // each unordered node pair evaluated once by the node that starts its circulant distance.
