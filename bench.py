@@ -46,17 +46,21 @@ def residual_bytes_per_launch(m, op):
 
 
 # Algorithmic FP64 flops of the 2D Ismail-Roe split-form residual, counted from
-# the kernel's arithmetic with fma = 2 and div = sqrt = log = pow = 1:
-#   two-point flux F*(q_j, q_k) in both parametric directions   71
-#     (two logavg on the series branch 2 x 14, Roe-like averages 15,
-#      two directions 2 x 11)
-#   res_j -= 2 S_jk F*, res_k += 2 S_jk F* for 4 variables x 2 directions  32
-# per node: IR z-variables 15; per face node: IRSLF flux (two z-variable sets,
-# single-direction IR flux, two entropy-variable conversions, A0, lambda_max,
-# lambda A0 dv) 230 plus the two-sided face integration 16.
-IR_FLOPS_PER_PAIR = 71 + 32
-IR_FLOPS_PER_NODE = 15
-IRSLF_FLOPS_PER_FACE_NODE = 230 + 16
+# the kernel's arithmetic with fma = 2 and div = sqrt = log = 1:
+#   per node pair (each unordered pair once):
+#     direction n = sum_d 2 S_d[j][k] dxidx_d                       6
+#     IR flux F*(q_j, q_k) . n                                      64
+#       (two logavg on the series branch 2 x 11, sums and reciprocals 4,
+#        logavg(z4) 3, averages and enthalpy 24, one direction 11)
+#     res_j -= F*, res_k += F*                                      8
+#   per node: IR z-variables 18 and the logs of z1 and z4 2          20
+#   per face node (each interface once): IRSLF flux with the neighbour's
+#     z-variables and logs 20, the one-direction IR flux 64, two entropy-variable
+#     conversions from the logs 46, jump and average 12, lambda_max A0 dv 61,
+#     and the two-sided face integration 8                          211
+IR_FLOPS_PER_PAIR = 6 + 64 + 8
+IR_FLOPS_PER_NODE = 20
+IRSLF_FLOPS_PER_FACE_NODE = 203 + 8
 
 
 def residual_flops_per_launch(m, op):
