@@ -212,6 +212,20 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   if (!st) st = upload(&c->d_nrm_bndry, m->nrm_bndry, nB * Nf * dim, sm);
   if (!st) st = upload(&c->d_coords_bndry, m->coords_bndry, nB * Nf * dim, sm);
   if (!st) st = upload(&c->d_Q, s->Q, (size_t)dim * Np * Np, sm);
+  {
+    // 2 S_d[a][a + dd mod Np] = Q_d[a][b] - Q_d[b][a] for dd = 0..Np/2, the
+    // order in which the split-form pair rounds read it
+    const int ns = Np / 2 + 1;
+    std::vector<double> s2c((size_t)dim * ns * Np);
+    for (int d = 0; d < dim; ++d)
+      for (int dd = 0; dd < ns; ++dd)
+        for (int a = 0; a < Np; ++a) {
+          const int b = (a + dd) % Np;
+          s2c[((size_t)d * ns + dd) * Np + a] = s->Q[((size_t)d * Np + a) * Np + b] - s->Q[((size_t)d * Np + b) * Np + a];
+        }
+    if (!st) st = upload(&c->d_S2c, s2c.data(), s2c.size(), sm);
+    if (!st) st = (hipStreamSynchronize(sm) == hipSuccess) ? PDES_OK : PDES_ERR_HIP;
+  }
   if (!st) st = upload(&c->d_interp, s->interp, (size_t)s->stencil * Nf, sm);
   if (!st) st = upload(&c->d_perm, s->perm, (size_t)s->stencil * nfaces, sm);
   if (!st) st = upload(&c->d_nbrperm, s->nbrperm, (size_t)Nf * s->norient, sm);
@@ -280,7 +294,7 @@ int pdes_ctx_destroy(pdes_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   jacobian_free(c);
   precond_free(c);
-  void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q,
+  void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q, c->d_S2c,
                   c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_q,
                   c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
                   c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale,

@@ -24,6 +24,7 @@ struct pdes_ctx {
   double* d_nrm_bndry = nullptr;
   double* d_coords_bndry = nullptr;
   double* d_Q = nullptr;
+  double* d_S2c = nullptr;    // 2 S by circulant distance (split-form volume)
   double* d_interp = nullptr;
   int32_t* d_perm = nullptr;
   int32_t* d_nbrperm = nullptr;
@@ -80,7 +81,7 @@ struct pdes_ctx {
   int64_t active_count() const { return nlist >= 0 ? nlist : numEl; }
   const int32_t* active_list() const { return nlist >= 0 ? d_el_list : nullptr; }
   pdes::DevSBP devsbp() const {
-    return pdes::DevSBP{d_Q, d_interp, d_perm, d_nbrperm, d_nodepos, d_wface, norient};
+    return pdes::DevSBP{d_Q, d_interp, d_perm, d_nbrperm, d_nodepos, d_wface, norient, d_S2c};
   }
 };
 

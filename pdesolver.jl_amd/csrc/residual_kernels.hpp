@@ -56,6 +56,7 @@ struct DevSBP {
   const int32_t* __restrict__ nodepos; // [i][f]: stencil position of node i on face f, -1 if none
   const double* __restrict__ wface;    // [k]
   int32_t norient;
+  const double* __restrict__ S2c;      // [d][dd][a] = Q_d[a][b] - Q_d[b][a], b = a + dd mod Np, dd <= Np/2
 };
 
 __device__ __forceinline__ int32_t launch_count(const DevMesh& m) { return m.el_list ? m.nlist : m.numEl; }
@@ -172,13 +173,8 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     // the split form only needs the antisymmetric part Q - Q^T = 2 S, and
     // only in the order the pair rounds read it: entry [d][dd][a] holds
     // 2 S_d[a][a+dd mod NP], so the lanes of one round read consecutive words
-    constexpr int NS = NP / 2 + 1;
-    for (int x = tid; x < DIM * NS * NP; x += blockDim.x) {
-      const int d = x / (NS * NP), dd = (x / NP) % NS, a = x % NP;
-      int b = a + dd;
-      if (b >= NP) b -= NP;
-      (&sm.Q[0][0][0])[x] = sbp.Q[(d * NP + a) * NP + b] - sbp.Q[(d * NP + b) * NP + a];
-    }
+    // (the table is built once per context, sbp.S2c)
+    for (int x = tid; x < DIM * (NP / 2 + 1) * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.S2c[x];
   } else {
     for (int x = tid; x < DIM * NP * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.Q[x];
   }
@@ -315,29 +311,27 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           const bool i_is_j = i > kk;
           const int jnode = i_is_j ? i : kk;
           const int knode = i_is_j ? kk : i;
-          double nrmD[DIM * DIM];   // [x][d]
-          if (mesh.metric_per_node && !i_is_j) {
-#pragma unroll
-            for (int x = 0; x < DIM; ++x)
-#pragma unroll
-              for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = metric<DIM, NP>(mesh, e, kk)[x * DIM + d];
-          } else {
-            // node j is this node, or the metric is the element's constant one
-#pragma unroll
-            for (int x = 0; x < DIM; ++x)
-#pragma unroll
-              for (int d = 0; d < DIM; ++d) nrmD[x * DIM + d] = dxi[x][d];
-          }
           // sum_d 2 S_d[i][kk] F_d is the IR flux in the one direction
-          // n = sum_d 2 S_d[i][kk] dxidx[d, :] (the flux is linear in the direction)
-          double n[DIM];
+          // n = sum_d 2 S_d[i][kk] dxidx[d, :] of node j (the flux is linear in
+          // the direction); node j is this node, or the metric is the
+          // element's constant one, except for per-node metrics with j = kk
+          double s2[DIM], n[DIM];
 #pragma unroll
-          for (int x = 0; x < DIM; ++x) n[x] = 0.0;
+          for (int d = 0; d < DIM; ++d) s2[d] = s2pl[(d * (NPAIR + 1) + dd) * NP + i];
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) {
-            const double s2 = s2pl[(d * (NPAIR + 1) + dd) * NP + i];
+          for (int x = 0; x < DIM; ++x) {
+            n[x] = s2[0] * dxi[x][0];
 #pragma unroll
-            for (int x = 0; x < DIM; ++x) n[x] = n[x] + s2 * nrmD[x * DIM + d];
+            for (int d = 1; d < DIM; ++d) n[x] = n[x] + s2[d] * dxi[x][d];
+          }
+          if (mesh.metric_per_node && !i_is_j) {
+            const double* mk = metric<DIM, NP>(mesh, e, kk);
+#pragma unroll
+            for (int x = 0; x < DIM; ++x) {
+              n[x] = s2[0] * mk[x * DIM];
+#pragma unroll
+              for (int d = 1; d < DIM; ++d) n[x] = n[x] + s2[d] * mk[x * DIM + d];
+            }
           }
           T zL[ND], zR[ND];
 #pragma unroll
