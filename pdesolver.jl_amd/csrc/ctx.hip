@@ -167,10 +167,13 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
     }
 
   // inverse mass matrix, calcMassMatrixInverse (src/Utils/mass_matrix.jl:20-55)
-  std::vector<double> Minv((size_t)c->numDof);
+  // (per DOF as the reference stores it, and once per node for the RK stage
+  // epilogue, which would otherwise read the same value ndof times)
+  std::vector<double> Minv((size_t)c->numDof), Minv_node((size_t)m->numEl * Np);
   for (int e = 0; e < m->numEl; ++e)
     for (int j = 0; j < Np; ++j) {
       double v = 1.0 / (s->w[j] / m->jac[(size_t)e * Np + j]);
+      Minv_node[(size_t)e * Np + j] = v;
       for (int k = 0; k < c->nd; ++k) Minv[((size_t)e * Np + j) * c->nd + k] = v;
     }
 
@@ -232,6 +235,7 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   if (!st) st = upload(&c->d_nodepos, c->h_nodepos.data(), c->h_nodepos.size(), sm);
   if (!st) st = upload(&c->d_wface, s->wface, (size_t)Nf, sm);
   if (!st) st = upload(&c->d_Minv, Minv.data(), Minv.size(), sm);
+  if (!st) st = upload(&c->d_Minv_node, Minv_node.data(), Minv_node.size(), sm);
   if (!st) st = upload(&c->d_ptc_diag, ptc.data(), ptc.size(), sm);
   if (!st) st = upload<double>(&c->d_ptc_scale, nullptr, 1, sm);
   if (!st) st = upload<double>(&c->d_q, nullptr, (size_t)c->numDof, sm);
@@ -271,7 +275,7 @@ int pdes_estimate_bytes(const pdes_mesh_desc* m, const pdes_sbp_desc* s, int fp6
   b += nel * nfaces * 16;                              // element -> face records
   b += (nI + 2 * nB) * Nf * dim * 8;                   // face normals, boundary normals and coordinates
   b += (dim * Np * Np + s->stencil * Nf + Nf) * 8 + (s->stencil * nfaces + Nf * s->norient + Np * nfaces) * 4;
-  b += ndof * 8 * 5;                                   // Minv, q, res, two work vectors
+  b += ndof * 8 * 5 + nel * Np * 8;                    // Minv (per DOF and per node), q, res, two work vectors
   b += nel * Np * 8;                                   // implicit-Euler diagonal
   b += (4096 + 64 + 8) * 8;                            // reduction scratch, scalars, flags
   // BSR Jacobian: element blocks plus the face couplings of every interface
@@ -295,7 +299,7 @@ int pdes_ctx_destroy(pdes_ctx* c) {
   jacobian_free(c);
   precond_free(c);
   void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q, c->d_S2c,
-                  c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_q,
+                  c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_Minv_node, c->d_q,
                   c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
                   c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale,
                   c->d_inst_active, c->d_qbc};

@@ -31,6 +31,7 @@ struct pdes_ctx {
   int32_t* d_nodepos = nullptr;
   double* d_wface = nullptr;
   double* d_Minv = nullptr;
+  double* d_Minv_node = nullptr;   // one value per volume node
   // host copies used by assembly / functionals
   std::vector<double> h_w, h_Q, h_interp, h_wface, h_jac, h_nrm_bndry, h_coords_bndry;
   std::vector<int32_t> h_perm, h_nbrperm, h_nodepos, h_interfaces, h_bndryfaces, h_bndry_offsets, h_bc_type;

@@ -128,7 +128,7 @@ int pdes_autotune(pdes_ctx* c, int iters, int32_t* chosen) {
     for (int var = 0; var < NUM_RES_VARIANTS; ++var) {
       c->res_variant[rkmode] = var;
       // stage-1 epilogue: reads q and the accumulator, writes scratch buffers only
-      RKStage rk{c->d_Minv, c->d_q, c->d_work, c->d_work2, 2.0, 0.0, 0.0, 1};
+      RKStage rk{c->d_Minv_node, c->d_q, c->d_work, c->d_work2, 2.0, 0.0, 0.0, 1};
       const RKStage* rkp = rkmode ? &rk : nullptr;
       int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);  // warm-up
       if (st) return st;
@@ -245,7 +245,7 @@ int pdes_rk4(pdes_ctx* c, int nsteps, double h, double* elapsed_ms) {
   double* qout[4] = {stage_a, stage_b, stage_a, q_old};
   for (int s = 0; s < nsteps; ++s) {
     for (int stage = 0; stage < 4; ++stage) {
-      RKStage rk{c->d_Minv, q_old, acc, qout[stage], wst[stage], cst[stage], h / 6, stage};
+      RKStage rk{c->d_Minv_node, q_old, acc, qout[stage], wst[stage], cst[stage], h / 6, stage};
       int st = launch_residual(c, qin[stage], nullptr, nullptr, nullptr, c->phys.check_positivity != 0, &rk);
       if (st) return st;
     }

@@ -116,7 +116,7 @@ enum { VOL_WEAK = 1, VOL_SPLIT = 2 };
 // storing res, k = Minv .* res updates the stage accumulator and writes the
 // next stage state (or, in the last stage, the new solution in place).
 struct RKStage {
-  const double* Minv;
+  const double* Minv;            // inverse mass matrix, one value per volume node
   const double* q_old;           // aliases q_next in the last stage (in-place update)
   double* acc;
   double* q_next;                // stage state written (stages 1-3) / q_old updated (stage 4)
@@ -717,7 +717,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     }
     if constexpr (RK) {
       const int64_t base = (e * NP + i) * ND;
-      const double minv = rk.Minv[base];
+      const double minv = rk.Minv[e * NP + i];
 #pragma unroll
       for (int c = 0; c < ND; ++c) {
         const double k = re(ri[c]) * minv;
