@@ -49,18 +49,19 @@ def residual_bytes_per_launch(m, op):
 # the kernel's arithmetic with fma = 2 and div = sqrt = log = 1:
 #   per node pair (each unordered pair once):
 #     direction n = sum_d 2 S_d[j][k] dxidx_d                       6
-#     IR flux F*(q_j, q_k) . n                                      64
-#       (two logavg on the series branch 2 x 11, sums and reciprocals 4,
-#        logavg(z4) 3, averages and enthalpy 24, one direction 11)
+#     IR flux F*(q_j, q_k) . n                                      59
+#       (two logavg on the series branch 2 x 12, sums, reciprocals and
+#        rho_hat 8, velocity average 4, p1_hat 1, |v|^2 3, h_hat 8,
+#        one direction 11)
 #     res_j -= F*, res_k += F*                                      8
 #   per node: IR z-variables 18 and the logs of z1 and z4 2          20
 #   per face node (each interface once): IRSLF flux with the neighbour's
-#     z-variables and logs 20, the one-direction IR flux 64, two entropy-variable
+#     z-variables and logs 20, the one-direction IR flux 59, two entropy-variable
 #     conversions from the logs 46, jump and average 12, lambda_max A0 dv 61,
-#     and the two-sided face integration 8                          211
-IR_FLOPS_PER_PAIR = 6 + 64 + 8
+#     and the two-sided face integration 8                          206
+IR_FLOPS_PER_PAIR = 6 + 59 + 8
 IR_FLOPS_PER_NODE = 20
-IRSLF_FLOPS_PER_FACE_NODE = 203 + 8
+IRSLF_FLOPS_PER_FACE_NODE = 198 + 8
 
 
 def residual_flops_per_launch(m, op):

@@ -168,17 +168,23 @@ template <class T, class LP>
 __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r, LP lgL, LP lgR) {
   const T f = (aL - aR) * r;
   const T u = f * f;
-  if (re(u) < 1e-3) return 1.0 + u * (1.0 / 3.0 + u * (1.0 / 5.0 + u * (1.0 / 7.0 + u * (1.0 / 9.0))));
+  if (re(u) < 1e-3) {
+    // 1 + u/3 + u^2/5 + u^3/7 + u^4/9, evaluated as (1 + u/3) + u^2 ((1/5 + u/7) + u^2/9)
+    const T u2 = u * u;
+    return (1.0 + u * (1.0 / 3.0)) + u2 * ((1.0 / 5.0 + u * (1.0 / 7.0)) + u2 * (1.0 / 9.0));
+  }
   if constexpr (std::is_same<LP, std::nullptr_t>::value) return dlog(aL / aR) * (0.5 * drcp(f));
   else return (*lgL - *lgR) * (0.5 * drcp(f));
 }
 
 // calcEulerFlux_IR, multi-direction  bc_solvers.jl:776-805 (2D), 842-874 (3D),
 // from precomputed z vectors.  dir: [x][ndir] ; F: [ndir][ndof].
-// The reference's averages, with its divisions shared: z1 sum s0 = zL0 + zR0
-// (r0 = 1/s0 serves as the reference's 1/(zL1 + zR1) and in logavg(z1)),
-// 1/logavg(z1) = 2 F0 r0, logavg(z4) = s3 / (2 F3), and
-// h_hat = (gamma/(gamma-1)) p2_hat / rho_hat with 1/rho_hat = 2 r0 (2 F3 r3).
+// The reference's averages with its divisions shared and its products
+// simplified: with s0 = zL1 + zR1, s3 = zL4 + zR4, r0 = 1/s0, r3 = 1/s3 and
+// the logavg factors F0, F3 (logavg(z1) = s0/(2 F0), logavg(z4) = s3/(2 F3)),
+//   rho_hat = z1_avg logavg(z4) = s0 s3 / (4 F3),   p1_hat = s3 r0,
+//   p2_hat = p1_hat (c1 F0/F3 + c2)   (c1 = (gamma+1)/(2 gamma), c2 = (gamma-1)/(2 gamma)),
+//   h_hat = gamma/(gamma-1) p2_hat/rho_hat + |v|^2/2 = 4 gamma/(gamma-1) r0^2 (c1 F0 + c2 F3) + |v|^2/2.
 // The three reciprocals (r0, r3, 1/F3) are drcp's Newton-refined estimates.
 // lzL / lzR: optional logs {log z1, log z4} of the two states (pointers to
 // log z1, with log z4 LS elements further; read only on the log branch of
@@ -200,18 +206,15 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
     F0 = logavg_factor(zL[0], zR[0], r0, lzL, lzR);
     F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, lzL + LS, lzR + LS);
   }
-  const T la = s3 * (0.5 * drcp(F3));       // logavg(z4_L, z4_R)
-  const T inv_la0 = 2.0 * F0 * r0;          // 1 / logavg(z1_L, z1_R)
-  const T rho_hat = 0.5 * s0 * la;
+  const T rho_hat = (0.25 * s0) * (s3 * drcp(F3));
   T vh[3];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) vh[d] = (zL[1 + d] + zR[1 + d]) * r0;
   const T p1_hat = s3 * r0;
-  const T p2_hat = ((gamma + 1) / (2 * gamma)) * la * inv_la0 + (gamma_1 / (2 * gamma)) * p1_hat;
   T ke = vh[0] * vh[0] + vh[1] * vh[1];
   if constexpr (DIM == 3) ke = ke + vh[2] * vh[2];
-  const T inv_rho_hat = 2.0 * r0 * (2.0 * F3 * r3);
-  const T h_hat = (gamma / gamma_1) * p2_hat * inv_rho_hat + 0.5 * ke;
+  const double c1 = (gamma + 1) / (2 * gamma), c2 = gamma_1 / (2 * gamma);
+  const T h_hat = ((4.0 * gamma / gamma_1) * (r0 * r0)) * (c1 * F0 + c2 * F3) + 0.5 * ke;
 #pragma unroll
   for (int i = 0; i < NDIR; ++i) {
     T un = dir[0 * NDIR + i] * vh[0] + dir[1 * NDIR + i] * vh[1];
