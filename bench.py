@@ -56,12 +56,13 @@ def residual_bytes_per_launch(m, op):
 #     res_j -= F*, res_k += F*                                      8
 #   per node: IR z-variables 18 and the logs of z1 and z4 2          20
 #   per face node (each interface once): IRSLF flux with the neighbour's
-#     z-variables and logs 20, the one-direction IR flux 59, two entropy-variable
-#     conversions from the logs 46, jump and average 12, lambda_max A0 dv 61,
-#     and the two-sided face integration 8                          206
+#     z-variables 18, the one-direction IR flux 59 and its two log ratios 8,
+#     two entropy-variable conversions without the entropy 36, the entropy
+#     jump from the log ratios 5, jump and average 12, lambda_max A0 dv 61,
+#     and the two-sided face integration 8                          207
 IR_FLOPS_PER_PAIR = 6 + 59 + 8
 IR_FLOPS_PER_NODE = 20
-IRSLF_FLOPS_PER_FACE_NODE = 198 + 8
+IRSLF_FLOPS_PER_FACE_NODE = 199 + 8
 
 
 def residual_flops_per_launch(m, op):

@@ -188,10 +188,12 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r,
 // The three reciprocals (r0, r3, 1/F3) are drcp's Newton-refined estimates.
 // lzL / lzR: optional logs {log z1, log z4} of the two states (pointers to
 // log z1, with log z4 LS elements further; read only on the log branch of
-// logavg), or nullptr.
+// logavg), or nullptr.  dl: optional output {log(zL1/zR1), log(zL4/zR4)},
+// which the logavg factors give without a log: log(aL/aR) = 2 F f.
 template <int DIM, int NDIR, class T, class LP = std::nullptr_t, int LS = 1>
 __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const T* zR,
-                                          const double* dir, T* F, LP lzL = nullptr, LP lzR = nullptr) {
+                                          const double* dir, T* F, LP lzL = nullptr, LP lzR = nullptr,
+                                          T* dl = nullptr) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
   const T s0 = zL[0] + zR[0];
@@ -205,6 +207,10 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
   } else {
     F0 = logavg_factor(zL[0], zR[0], r0, lzL, lzR);
     F3 = logavg_factor(zL[ND - 1], zR[ND - 1], r3, lzL + LS, lzR + LS);
+  }
+  if (dl) {
+    dl[0] = (2.0 * F0) * ((zL[0] - zR[0]) * r0);
+    dl[1] = (2.0 * F3) * ((zL[ND - 1] - zR[ND - 1]) * r3);
   }
   const T rho_hat = (0.25 * s0) * (s3 * drcp(F3));
   T vh[3];
@@ -227,11 +233,13 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
   }
 }
 
-// convertToIR_  src/solver/euler/conversion.jl:160-205.  With LZ set, the
-// entropy s = log(p) - gamma log(rho) comes from the logs of the IR parameter
-// vector, log rho = log z1 + log z4 and log p = log z4 - log z1 (lz1 and
-// lz4 given), instead of two logs of the state.
-template <int DIM, class T, bool LZ = false>
+// convertToIR_  src/solver/euler/conversion.jl:160-205.  The entropy
+// s = log(p) - gamma log(rho) enters only v[0], as -s/(gamma-1).  SMODE 0
+// takes it from two logs of the state; SMODE 1 from the logs of the IR
+// parameter vector, log rho = log z1 + log z4 and log p = log z4 - log z1
+// (lz1 and lz4 given); SMODE 2 leaves it out (s = 0), for callers that add
+// the entropy jump of two states themselves.
+template <int DIM, class T, int SMODE = 0>
 __device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* qe, T lz1 = 0.0, T lz4 = 0.0) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1, gamma_1i = 1.0 / gamma_1;
@@ -241,7 +249,8 @@ __device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* q
   T rho_int = qc[ND - 1] - k1;
   // log(gamma_1 rho_int / rho^gamma) taken as log(gamma_1 rho_int) - gamma log(rho)
   T s;
-  if constexpr (LZ) s = (lz4 - lz1) - gamma * (lz1 + lz4);
+  if constexpr (SMODE == 1) s = (lz4 - lz1) - gamma * (lz1 + lz4);
+  else if constexpr (SMODE == 2) s = 0.0;
   else s = dlog(gamma_1 * rho_int) - gamma * dlog(qc[0]);
   T fac = drcp(rho_int);
   T tmp1 = -qc[0] * fac * gamma_1i;
@@ -312,19 +321,25 @@ __device__ __forceinline__ void irslf_flux(const PhysConst& p, const T* qL, cons
   lf_entropy_apply<DIM>(p, qa, dv, nrm, 1.0, F);
 }
 
-// irslf_flux from the IR parameter vectors of both states and their logs
-// (lzL, lzR point to log z1, with log z4 LS elements further), for kernels
-// that hold them per node already: no square roots or logs of the own state.
-template <int DIM, class T, int LS = 1>
+// irslf_flux from the IR parameter vectors of both states, for kernels that
+// hold them per node already: no square roots of the own state, and no logs.
+// The entropy-variable jump is v(qL) - v(qR) without the entropy terms, plus
+// -(s_L - s_R)/(gamma-1) with s_L - s_R = log(p_L/p_R) - gamma log(rho_L/rho_R)
+// = (1 - gamma) log(zL4/zR4) - (1 + gamma) log(zL1/zR1), log ratios that the
+// IR flux's logavg factors provide.
+template <int DIM, class T>
 __device__ __forceinline__ void irslf_flux_z(const PhysConst& p, const T* qL, const T* qR, const T* zL,
-                                             const T* zR, const T* lzL, const T* lzR, const double* nrm, T* F) {
+                                             const T* zR, const double* nrm, T* F) {
   constexpr int ND = DIM + 2;
-  ir_flux_z<DIM, 1, T, const T*, LS>(p, zL, zR, nrm, F, lzL, lzR);
+  const double gamma = p.gamma;
+  T dl[2];
+  ir_flux_z<DIM, 1>(p, zL, zR, nrm, F, nullptr, nullptr, dl);
   T dv[ND], vR[ND];
-  to_ir_vars<DIM, T, true>(p, qL, dv, lzL[0], lzL[LS]);
-  to_ir_vars<DIM, T, true>(p, qR, vR, lzR[0], lzR[LS]);
+  to_ir_vars<DIM, T, 2>(p, qL, dv);
+  to_ir_vars<DIM, T, 2>(p, qR, vR);
 #pragma unroll
   for (int i = 0; i < ND; ++i) dv[i] = dv[i] - vR[i];
+  dv[0] = dv[0] - ((1.0 - gamma) * dl[1] - (1.0 + gamma) * dl[0]) * (1.0 / p.gamma_1);
   T qa[ND];
 #pragma unroll
   for (int i = 0; i < ND; ++i) qa[i] = 0.5 * (qL[i] + qR[i]);

@@ -419,26 +419,18 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           if constexpr (FLUX == PDES_FLUX_ROE) {
             roe_flux<DIM>(phys, qL, qR, nrm, Fk);
           } else if constexpr (VOL == VOL_SPLIT) {
-            // the own node's z and logs are in the LDS planes already; the
-            // neighbour's are computed here by the same functions, so the
-            // two elements of the face evaluate the same bits
-            T zn[ND], zL[ND], zR[ND], ln[2], lL[2], lR[2];
+            // the own node's z-variables are in the LDS planes already; the
+            // neighbour's are computed here by the same function, so the two
+            // elements of the face evaluate the same bits
+            T zn[ND], zL[ND], zR[ND];
             ir_zvars<DIM>(phys, qn, zn);
-            ln[0] = dlog(zn[0]);
-            ln[1] = dlog(zn[ND - 1]);
 #pragma unroll
             for (int c = 0; c < ND; ++c) {
               const T zo = zpl[c * PL + me + node];
               zL[c] = side == 0 ? zo : zn[c];
               zR[c] = side == 0 ? zn[c] : zo;
             }
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-              const T lo = lpl[c * PL + me + node];
-              lL[c] = side == 0 ? lo : ln[c];
-              lR[c] = side == 0 ? ln[c] : lo;
-            }
-            irslf_flux_z<DIM>(phys, qL, qR, zL, zR, lL, lR, nrm, Fk);
+            irslf_flux_z<DIM>(phys, qL, qR, zL, zR, nrm, Fk);
           } else {
             irslf_flux<DIM>(phys, qL, qR, nrm, Fk);
           }
