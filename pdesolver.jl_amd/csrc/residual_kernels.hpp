@@ -127,21 +127,20 @@ struct RKStage {
 template <class Op, int VOL, int FLUX, int EPB, class T>
 struct ResidualSmem {
   static constexpr int NP = Op::NP, ND = Op::ND, DIM = Op::DIM, NF = Op::NF, NFACES = Op::NFACES;
-  double Q[DIM][NP][NP];
+  // weak form: Q_d[j][i]; split form: 2 S by circulant distance, [d][dd][i]
+  double Q[(VOL == VOL_SPLIT) ? DIM * (NP / 2 + 1) * NP : DIM * NP * NP];
   double interp[Op::ST][NF];
   int32_t perm[Op::ST][NFACES];
   int32_t nodepos[NP][NFACES];
   double wface[NF];
   T q[EPB][NP][ND];
   // weak form: volume fluxes; split form: IR z-variables and partner contributions
-  // (the F, C and Lz arrays are planes of one value per (element, node),
+  // (the F and C arrays are planes of one value per (element, node),
   //  [component][element][node], so that the lanes of a pair round, which
   //  read node i + dd, touch consecutive words)
   T F[EPB][(VOL == VOL_WEAK) ? DIM * NP * ND : NP * ND];
   // split form: partner contributions, two buffers used in alternate rounds
   T C[(VOL == VOL_SPLIT) ? 2 : 1][EPB][(VOL == VOL_SPLIT) ? NP * ND : 1];
-  // split form: per-node logs of the IR variables z1 and z4 (for logavg)
-  T Lz[EPB][(VOL == VOL_SPLIT) ? NP : 1][2];
   // entropy-stable element face integral (dense faces): own IR entropy
   // variables, the neighbours' stencil z and entropy variables per face, and
   // the entropy-conservative sums per (face, own stencil node)
@@ -151,10 +150,14 @@ struct ResidualSmem {
   T Wn[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
   T H[EPB][ES ? NFACES : 1][ES ? Op::ST : 1][ND];
   // face contributions; the weak form keeps them in the flux planes F, which
-  // are dead once the volume term is summed (one more barrier, less LDS)
+  // are dead once the volume term is summed (one more barrier, less LDS); the
+  // split form keeps them, where they fit, in the partner buffer that the last
+  // pair round does not use (its last reads precede that round's barrier)
   static constexpr bool G_IN_F = (VOL == VOL_WEAK);
+  static constexpr bool G_IN_C = (VOL == VOL_SPLIT) && (NP >= NFACES * NF);
+  static constexpr int G_C_BUF = ((NP / 2) + 1) & 1;
   static_assert(!G_IN_F || DIM * NP >= NFACES * NF, "face contributions must fit the weak-form flux planes");
-  T G[G_IN_F ? 1 : EPB][NFACES][NF][ND];
+  T G[(G_IN_F || G_IN_C) ? 1 : EPB][NFACES][NF][ND];
 };
 
 template <class Op, int VOL, int FLUX, int EPB, class T, bool RK = false, int WAVES = 2>
@@ -174,9 +177,9 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     // only in the order the pair rounds read it: entry [d][dd][a] holds
     // 2 S_d[a][a+dd mod NP], so the lanes of one round read consecutive words
     // (the table is built once per context, sbp.S2c)
-    for (int x = tid; x < DIM * (NP / 2 + 1) * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.S2c[x];
+    for (int x = tid; x < DIM * (NP / 2 + 1) * NP; x += blockDim.x) sm.Q[x] = sbp.S2c[x];
   } else {
-    for (int x = tid; x < DIM * NP * NP; x += blockDim.x) (&sm.Q[0][0][0])[x] = sbp.Q[x];
+    for (int x = tid; x < DIM * NP * NP; x += blockDim.x) sm.Q[x] = sbp.Q[x];
   }
   for (int x = tid; x < ST * NF; x += blockDim.x) (&sm.interp[0][0])[x] = sbp.interp[x];
   for (int x = tid; x < ST * NFACES; x += blockDim.x) (&sm.perm[0][0])[x] = sbp.perm[x];
@@ -187,14 +190,12 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   const int i = tid - el * NP;
   // LDS planes: value c of node n of this block's element el at
   // [c * PL + el * NP + n] (weak form: component k of the flux in parametric
-  // direction d as c = d * ND + k; split form: IR parameter vector z in zpl,
-  // log z1 / log z4 in lpl)
+  // direction d as c = d * ND + k; split form: IR parameter vector z)
   constexpr int PL = EPB * NP;
   T* const zpl = &sm.F[0][0];
-  T* const lpl = &sm.Lz[0][0][0];
   const int me = el * NP;
   // face contributions, planes [c][element][face][face node]
-  T* const gpl = Smem::G_IN_F ? &sm.F[0][0] : &sm.G[0][0][0][0];
+  T* const gpl = Smem::G_IN_F ? &sm.F[0][0] : Smem::G_IN_C ? &sm.C[Smem::G_C_BUF][0][0] : &sm.G[0][0][0][0];
   const int64_t eslot = (int64_t)blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (eslot < launch_count(mesh));
   const int64_t e = active ? element_of(mesh, eslot) : 0;
@@ -270,7 +271,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       for (int d = 0; d < DIM; ++d)
 #pragma unroll 4
         for (int j = 0; j < NP; ++j) {
-          double qji = sm.Q[d][j][i];
+          double qji = sm.Q[(d * NP + j) * NP + i];
 #pragma unroll
           for (int k = 0; k < ND; ++k) ri[k] = ri[k] + qji * zpl[(d * ND + k) * PL + me + j];
         }
@@ -284,8 +285,6 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       ir_zvars<DIM>(phys, qi, z);
 #pragma unroll
       for (int k = 0; k < ND; ++k) zpl[k * PL + me + i] = z[k];
-      lpl[me + i] = dlog(z[0]);
-      lpl[PL + me + i] = dlog(z[ND - 1]);
     }
     __syncthreads();
     // circulant distance dd: node i evaluates the pair (i, i+dd mod NP); the
@@ -294,7 +293,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     // suffices: a thread writes a buffer again only after the next round's
     // barrier, which every thread passes after its reads of that buffer.
     constexpr int NPAIR = NP / 2;
-    const double* const s2pl = &sm.Q[0][0][0];   // 2 S by circulant distance, [d][dd][i]
+    const double* const s2pl = sm.Q;   // 2 S by circulant distance, [d][dd][i]
     for (int dd = 1; dd <= NPAIR; ++dd) {
       const bool half = (2 * dd == NP);            // even NP: distance NP/2 owned by i < NP/2
       T* const cbuf = &sm.C[dd & 1][0][0];
@@ -339,7 +338,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
             zL[k] = zpl[k * PL + me + jnode];
             zR[k] = zpl[k * PL + me + knode];
           }
-          ir_flux_z<DIM, 1, T, const T*, PL>(phys, zL, zR, n, g, lpl + me + jnode, lpl + me + knode);
+          ir_flux_z<DIM, 1>(phys, zL, zR, n, g);
           // res_i -= 2 S[i][kk] F, and the partner gets -2 S[kk][i] F = +g
 #pragma unroll
           for (int k = 0; k < ND; ++k) ri[k] = ri[k] - g[k];
