@@ -54,14 +54,14 @@ def residual_bytes_per_launch(m, op):
 #        rho_hat 8, velocity average 4, p1_hat 1, |v|^2 3, h_hat 8,
 #        one direction 11)
 #     res_j -= F*, res_k += F*                                      8
-#   per node: IR z-variables 18 and the logs of z1 and z4 2          20
+#   per node: IR z-variables                                        18
 #   per face node (each interface once): IRSLF flux with the neighbour's
 #     z-variables 18, the one-direction IR flux 59 and its two log ratios 8,
 #     two entropy-variable conversions without the entropy 36, the entropy
 #     jump from the log ratios 5, jump and average 12, lambda_max A0 dv 61,
 #     and the two-sided face integration 8                          207
 IR_FLOPS_PER_PAIR = 6 + 59 + 8
-IR_FLOPS_PER_NODE = 20
+IR_FLOPS_PER_NODE = 18
 IRSLF_FLOPS_PER_FACE_NODE = 199 + 8
 
 
