@@ -340,6 +340,85 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
   else x[e * N + r] -= (double)y;
 }
 
+// The same level with LPR lanes per row, for dense face stencils (a row has
+// tens of neighbour blocks): lane l of row r takes the blocks q0 + l,
+// q0 + l + LPR, ... and the columns cc = l, l + LPR, ... of D'_e^-1; the
+// partial sums meet in LDS and are added in lane order, so the bits do not
+// depend on the level's width or on the other instances of a batch.
+template <int NP, int ND, class BT, int LPR>
+__global__ void __launch_bounds__(256)
+ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, const int64_t* __restrict__ rowptr,
+                          const int32_t* __restrict__ colind, const int32_t* __restrict__ own,
+                          const BT* __restrict__ vals, const float* __restrict__ inv,
+                          const double* __restrict__ b, double* __restrict__ x, int forward,
+                          const int32_t* __restrict__ el_inst, const int32_t* __restrict__ mask,
+                          const int32_t* __restrict__ inst_active) {
+  constexpr int N = NP * ND;
+  constexpr int TPE = N * LPR;
+  static_assert(TPE <= 256, "one element's rows must fit a workgroup");
+  constexpr int EPB = 256 / TPE;
+  __shared__ double tp[EPB][N][LPR];
+  __shared__ float yp[EPB][N][LPR];
+  __shared__ float v[EPB][N];
+  const int slot = threadIdx.x / TPE;
+  const int t = threadIdx.x - slot * TPE;
+  const int r = t / LPR, l = t - r * LPR;
+  const int64_t s = (int64_t)blockIdx.x * EPB + slot;
+  bool act = slot < EPB && s < count;
+  int64_t e = 0;
+  if (act) {
+    e = lev_el[s];
+    const int inst = el_inst ? el_inst[e] : 0;
+    if ((mask && !mask[inst]) || (inst_active && !inst_active[inst])) act = false;
+  }
+  if (act) {
+    const int i = r / ND, a = r - i * ND;
+    const int64_t row = e * NP + i;
+    const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
+    const int64_t q0 = forward ? p0 : p0 + own[row] + NP;
+    const int64_t q1 = forward ? p0 + own[row] : p1;
+    double acc = 0.0;
+#pragma unroll 2
+    for (int64_t p = q0 + l; p < q1; p += LPR) {
+      const int32_t col = colind[p];
+      const BT* blk = vals + p * ND * ND + a * ND;
+      const double* xc = x + (int64_t)col * ND;
+#pragma unroll
+      for (int c = 0; c < ND; ++c) acc += (double)blk[c] * xc[c];
+    }
+    tp[slot][r][l] = acc;
+  }
+  __syncthreads();
+  if (act && l == 0) {
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < LPR; ++k) sum += tp[slot][r][k];
+    v[slot][r] = (float)(forward ? b[e * N + r] - sum : sum);
+  }
+  __syncthreads();
+  if (act) {
+    const float* Mi = inv + e * N * N;
+    float y = 0.0f;
+    for (int cc = l; cc < N; cc += LPR) y += Mi[cc * N + r] * v[slot][cc];
+    yp[slot][r][l] = y;
+  }
+  __syncthreads();
+  if (act && l == 0) {
+    float y = 0.0f;
+#pragma unroll
+    for (int k = 0; k < LPR; ++k) y += yp[slot][r][k];
+    if (forward) x[e * N + r] = (double)y;
+    else x[e * N + r] -= (double)y;
+  }
+}
+
+// lanes per row of the sweeps: 1 for diagonal-E operators (a row meets at
+// most one neighbour node per face), up to 8 for dense face stencils
+template <int N, bool DIAGE>
+constexpr int ilu_lanes() {
+  return DIAGE ? 1 : (256 / N >= 8 ? 8 : 256 / N);
+}
+
 // dependency levels of the natural-order sweeps (host, once per context)
 static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
   const int nel = c->numEl, nf = c->nfaces;
@@ -422,11 +501,13 @@ static int ilu_factor(pdes_ctx* c) {
   return PDES_OK;
 }
 
-template <int NP, int ND>
+template <int NP, int ND, bool DIAGE>
 static int ilu_apply(pdes_ctx* c, const double* b, double* x) {
   pdes_jacobian* j = c->jac;
   pdes_precond* p = c->pc_obj;
-  constexpr int EPB = 256 / (NP * ND);
+  constexpr int N = NP * ND;
+  constexpr int LPR = ilu_lanes<N, DIAGE>();
+  constexpr int EPB = 256 / (N * LPR);
   for (int pass = 0; pass < 2; ++pass) {
     const std::vector<int64_t>& ptr = pass == 0 ? p->fptr : p->bptr;
     const int32_t* els = pass == 0 ? p->d_fel : p->d_bel;
@@ -434,6 +515,18 @@ static int ilu_apply(pdes_ctx* c, const double* b, double* x) {
       const int64_t cnt = ptr[l + 1] - ptr[l];
       if (cnt == 0) continue;
       const dim3 grid((unsigned)ceil_div(cnt, EPB));
+      if constexpr (LPR > 1) {
+        const dim3 block((unsigned)(EPB * N * LPR));
+        if (j->fp64)
+          hipLaunchKernelGGL((ilu_sweep_level_ml_kernel<NP, ND, double, LPR>), grid, block, 0, c->stream, els + ptr[l],
+                             cnt, j->d_rowptr, j->d_colind, p->d_own, (const double*)j->d_vals, p->d_inv, b, x,
+                             pass == 0, c->d_el_inst, c->d_mask, c->d_inst_active);
+        else
+          hipLaunchKernelGGL((ilu_sweep_level_ml_kernel<NP, ND, float, LPR>), grid, block, 0, c->stream, els + ptr[l],
+                             cnt, j->d_rowptr, j->d_colind, p->d_own, (const float*)j->d_vals, p->d_inv, b, x,
+                             pass == 0, c->d_el_inst, c->d_mask, c->d_inst_active);
+        continue;
+      }
       if (j->fp64)
         hipLaunchKernelGGL((ilu_sweep_level_kernel<NP, ND, double>), grid, dim3(256), 0, c->stream, els + ptr[l], cnt,
                            j->d_rowptr, j->d_colind, p->d_own, (const double*)j->d_vals, p->d_inv, b, x, pass == 0,
@@ -513,7 +606,7 @@ static int ilu_factor_dispatch(pdes_ctx* c) {
 
 static int ilu_apply_dispatch(pdes_ctx* c, const double* b, double* x) {
 #define PDES_ILU_A(D_, NP_, NF_, NS_, DE_) \
-  if (c->dim == D_ && c->Np == NP_ && (c->diag_e != 0) == DE_) return ilu_apply<NP_, D_ + 2>(c, b, x);
+  if (c->dim == D_ && c->Np == NP_ && (c->diag_e != 0) == DE_) return ilu_apply<NP_, D_ + 2, DE_>(c, b, x);
   PDES_FOR_EACH_SHAPE(PDES_ILU_A)
 #undef PDES_ILU_A
   set_error("no ILU(0) kernel for this operator");
