@@ -82,6 +82,9 @@ static int launch_jac_shape(pdes_ctx* c) {
   if constexpr (Op::DIAGE && Op::DIM == 2) {
     if (vol == 2 && flux == PDES_FLUX_IRSLF) return launch_jac<Op, VOL_SPLIT, PDES_FLUX_IRSLF, BT>(c);
   }
+  if constexpr (!Op::DIAGE) {
+    if (vol == 2 && flux == PDES_FLUX_ESLF) return launch_jac<Op, VOL_SPLIT, PDES_FLUX_ESLF, BT>(c);
+  }
   set_error("unsupported physics combination for the explicit Jacobian");
   return PDES_ERR_UNSUPPORTED;
 }
@@ -339,9 +342,6 @@ int assemble_coloring(pdes_ctx* c) {
 
 int assemble(pdes_ctx* c) {
   PDES_CHECK(c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
-  // no element-wise derivative kernel for the entropy-stable element face
-  // integral: assemble it by colouring
-  if (c->phys.face_flux == PDES_FLUX_ESLF) return assemble_coloring(c);
   int st = launch_assemble(c);
   if (st) return st;
   c->jac->assembled = true;

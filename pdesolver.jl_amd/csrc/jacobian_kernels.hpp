@@ -54,6 +54,152 @@ __device__ __forceinline__ int64_t bsr_find(const int64_t* __restrict__ rowptr, 
   return -1;
 }
 
+// dw/dq of the IR entropy variables at state q (convertToIR_ differentiated)
+template <int DIM>
+__device__ __forceinline__ void ir_vars_jacobian(const PhysConst& p, const double* q, double (*W)[DIM + 2]) {
+  constexpr int ND = DIM + 2;
+  dual<ND> qd[ND], w[ND];
+#pragma unroll
+  for (int c = 0; c < ND; ++c) { qd[c] = dual<ND>(q[c]); qd[c].d[c] = 1.0; }
+  to_ir_vars<DIM>(p, qd, w);
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) W[a][c] = w[a].d[c];
+}
+
+// E_tu[d] = sum_k R[iL][k] R[jR][kR(k)] B[k] n_d[k] of the entropy-conservative
+// face integral (calcECFaceIntegral, faceElementIntegrals.jl:58-120), with
+// (iL, jR) = (t, u) when this element is elementL and (u, t) otherwise
+template <class Op, class Sm>
+__device__ __forceinline__ void ec_weights(const Sm& sm, const DevSBP& sbp, const DevMesh& mesh, const ElFace& r,
+                                           int t, int u, double* E) {
+  constexpr int DIM = Op::DIM, NF = Op::NF;
+  const int o = (r.z >> 8) & 0xff, side = (r.z >> 16) & 1;
+  const int iL = side == 0 ? t : u, jR = side == 0 ? u : t;
+  const double* nr = mesh.nrm_face + (int64_t)r.y * NF * DIM;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) E[d] = 0.0;
+  for (int k = 0; k < NF; ++k) {
+    const int kRk = sbp.nbrperm[k * sbp.norient + o];
+    const double base = sm.interp[iL][k] * sm.interp[jR][kRk] * sm.wface[k];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) E[d] += base * nr[k * DIM + d];
+  }
+}
+
+// d/dq of F_IR(q_L, q_R) . E for the pair (own stencil node t, neighbour
+// stencil node u) of face f, with respect to the own state (wrt_own) or the
+// neighbour's; the sum over directions of E_d F_d is the flux in direction E
+template <class Op, class Sm>
+__device__ __forceinline__ void ec_pair_derivative(const Sm& sm, const DevSBP& sbp, const DevMesh& mesh,
+                                                   const PhysConst& phys, const double* qin, int f, int t, int u,
+                                                   bool wrt_own, double (*D)[Op::ND]) {
+  constexpr int DIM = Op::DIM, NP = Op::NP, ND = Op::ND;
+  const ElFace r = sm.rec[f];
+  const int fN = r.z & 0xff, side = (r.z >> 16) & 1;
+  const double* qn = qin + ((int64_t)r.x * NP + sm.perm[u][fN]) * ND;
+  dual<ND> qo[ND], qb[ND], zo[ND], zb[ND], zL[ND], zR[ND], F[ND];
+#pragma unroll
+  for (int c = 0; c < ND; ++c) {
+    qo[c] = dual<ND>(sm.q[sm.perm[t][f]][c]);
+    qb[c] = dual<ND>(qn[c]);
+    if (wrt_own) qo[c].d[c] = 1.0;
+    else qb[c].d[c] = 1.0;
+  }
+  ir_zvars<DIM>(phys, qo, zo);
+  ir_zvars<DIM>(phys, qb, zb);
+#pragma unroll
+  for (int c = 0; c < ND; ++c) {
+    zL[c] = side == 0 ? zo[c] : zb[c];
+    zR[c] = side == 0 ? zb[c] : zo[c];
+  }
+  double E[DIM];
+  ec_weights<Op>(sm, sbp, mesh, r, t, u, E);
+  ir_flux_z<DIM, 1>(phys, zL, zR, E, F);
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) D[a][c] = F[a].d[c];
+}
+
+// ESLF interior face f, own block (i, j): the entropy penalty through the
+// interpolated own entropy variables, sum_k sg R[si][ko] R[sj][ko] dP_k/dw_own
+// times dw/dq(q_j); for i == j also the entropy-conservative pairs of node si
+template <class Op, class Sm>
+__device__ __forceinline__ void eslf_own_block(const Sm& sm, const DevSBP& sbp, const DevMesh& mesh,
+                                               const PhysConst& phys, const double* qin, int f, int i, int j,
+                                               int si, int sj, double (*B)[Op::ND]) {
+  constexpr int DIM = Op::DIM, NF = Op::NF, ST = Op::ST, ND = Op::ND;
+  const double sg = sm.sgn[f];
+  double M[ND][ND], W[ND][ND];
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) M[a][c] = 0.0;
+  for (int k = 0; k < NF; ++k) {
+    const int m = sm.kown[f][k];
+    const double wgt = sg * sm.interp[si][m] * sm.interp[sj][m];
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < ND; ++c) M[a][c] += wgt * sm.dFo[f][k][a][c];
+  }
+  ir_vars_jacobian<DIM>(phys, sm.q[j], W);
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < ND; ++b) acc += M[a][b] * W[b][c];
+      B[a][c] += acc;
+    }
+  if (i != j) return;
+  for (int u = 0; u < ST; ++u) {
+    double D[ND][ND];
+    ec_pair_derivative<Op>(sm, sbp, mesh, phys, qin, f, si, u, true, D);
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < ND; ++c) B[a][c] += sg * D[a][c];
+  }
+}
+
+// ESLF interior face f, neighbour block (row node with stencil position si,
+// neighbour node jn with stencil position sj on the neighbour's face)
+template <class Op, class Sm>
+__device__ __forceinline__ void eslf_nbr_block(const Sm& sm, const DevSBP& sbp, const DevMesh& mesh,
+                                               const PhysConst& phys, const double* qin, int f, int jn,
+                                               int si, int sj, double (*B)[Op::ND]) {
+  constexpr int DIM = Op::DIM, NP = Op::NP, NF = Op::NF, ND = Op::ND;
+  const double sg = sm.sgn[f];
+  double M[ND][ND], W[ND][ND];
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) M[a][c] = 0.0;
+  for (int k = 0; k < NF; ++k) {
+    const double wgt = sg * sm.interp[si][sm.kown[f][k]] * sm.interp[sj][sm.knbr[f][k]];
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = 0; c < ND; ++c) M[a][c] += wgt * sm.dFn[f][k][a][c];
+  }
+  ir_vars_jacobian<DIM>(phys, qin + ((int64_t)sm.rec[f].x * NP + jn) * ND, W);
+  double D[ND][ND];
+  ec_pair_derivative<Op>(sm, sbp, mesh, phys, qin, f, si, sj, false, D);
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < ND; ++b) acc += M[a][b] * W[b][c];
+      B[a][c] += acc + sg * D[a][c];
+    }
+}
+
 template <class Op, int VOL, int FLUX, class BT>
 __global__ void __launch_bounds__(256)
 euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __restrict__ qin,
@@ -135,6 +281,57 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
       for (int a = 0; a < ND; ++a)
 #pragma unroll
         for (int c = 0; c < ND; ++c) { sm.dFo[f][k][a][c] = F[a].d[c]; sm.dFn[f][k][a][c] = 0.0; }
+    } else if constexpr (FLUX == PDES_FLUX_ESLF) {
+      // entropy penalty at face node k (calcEntropyPenaltyIntegral,
+      // faceElementIntegrals.jl:199-290) as a function of the interpolated
+      // IR entropy variables wl, wr: dP/dw_own and dP/dw_nbr; the chain
+      // through dw/dq of each stencil node is applied in the block tasks
+      const int fN = r.z & 0xff;
+      const int side = (r.z >> 16) & 1;
+      const int o = (r.z >> 8) & 0xff;
+      const int kR = sbp.nbrperm[k * sbp.norient + o];
+      const double* qnb = qin + r.x * NP * ND;
+      double wlv[ND], wrv[ND];
+#pragma unroll
+      for (int c = 0; c < ND; ++c) { wlv[c] = 0.0; wrv[c] = 0.0; }
+      for (int sn = 0; sn < ST; ++sn) {
+        double qo[ND], qn[ND], wo[ND], wn[ND];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+          qo[c] = sm.q[sm.perm[sn][f]][c];
+          qn[c] = qnb[sm.perm[sn][fN] * ND + c];
+        }
+        to_ir_vars<DIM>(phys, qo, wo);
+        to_ir_vars<DIM>(phys, qn, wn);
+        const double rl = sm.interp[sn][k], rr = sm.interp[sn][kR];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) {
+          wlv[c] += rl * (side == 0 ? wo[c] : wn[c]);
+          wrv[c] += rr * (side == 0 ? wn[c] : wo[c]);
+        }
+      }
+      dual<2 * ND> wl[ND], wr[ND], qa[ND], qlr[ND], dw[ND];
+#pragma unroll
+      for (int c = 0; c < ND; ++c) {
+        wl[c] = dual<2 * ND>(wlv[c]);
+        wl[c].d[c] = 1.0;
+        wr[c] = dual<2 * ND>(wrv[c]);
+        wr[c].d[ND + c] = 1.0;
+      }
+      ir_to_conservative<DIM>(phys, wl, qa);
+      ir_to_conservative<DIM>(phys, wr, qlr);
+#pragma unroll
+      for (int c = 0; c < ND; ++c) {
+        qa[c] = 0.5 * (qa[c] + qlr[c]);
+        dw[c] = wl[c] - wr[c];
+        F[c] = dual<2 * ND>(0.0);
+      }
+      lf_entropy_apply<DIM>(phys, qa, dw, mesh.nrm_face + ((int64_t)r.y * NF + k) * DIM, sm.wface[k], F);
+      const int oo = side == 0 ? 0 : ND, on = side == 0 ? ND : 0;
+#pragma unroll
+      for (int a = 0; a < ND; ++a)
+#pragma unroll
+        for (int c = 0; c < ND; ++c) { sm.dFo[f][k][a][c] = F[a].d[oo + c]; sm.dFn[f][k][a][c] = F[a].d[on + c]; }
     } else {
       const int64_t nb = r.x;
       const int fN = r.z & 0xff;
@@ -249,6 +446,8 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
           for (int a = 0; a < ND; ++a)
 #pragma unroll
             for (int c = 0; c < ND; ++c) B[a][c] += wgt * sm.dFo[f][k][a][c];
+        } else if (FLUX == PDES_FLUX_ESLF && sm.rec[f].x >= 0) {
+          if constexpr (FLUX == PDES_FLUX_ESLF) eslf_own_block<Op>(sm, sbp, mesh, phys, qin, f, i, j, si, sj, B);
         } else {
           for (int k = 0; k < NF; ++k) {
             const int m = sm.kown[f][k];
@@ -290,7 +489,9 @@ euler_jacobian_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __
         row = e * NP + i;
         col = (int64_t)r.x * NP + jn;
         const int si = sm.nodepos[i][f], sj = sbp.nodepos[jn * NFACES + fN];
-        if (si >= 0 && sj >= 0) {
+        if constexpr (FLUX == PDES_FLUX_ESLF) {
+          if (si >= 0 && sj >= 0) eslf_nbr_block<Op>(sm, sbp, mesh, phys, qin, f, jn, si, sj, B);
+        } else if (si >= 0 && sj >= 0) {
           for (int k = 0; k < NF; ++k) {
             const int m = sm.kown[f][k], mn = sm.knbr[f][k];
             const double wgt = sg * sm.wface[m] * sm.interp[si][m] * sm.interp[sj][mn];

@@ -535,9 +535,6 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
       // t of this element on face f against every stencil node of the
       // neighbour, E_ij^d = sum_k R[i][k] R[j][kR] B[k] n_d[k], IR flux in
       // the x, y (, z) directions
-      double ident[DIM * DIM];
-#pragma unroll
-      for (int a = 0; a < DIM * DIM; ++a) ident[a] = (a % (DIM + 1) == 0) ? 1.0 : 0.0;
       for (int tt = i; tt < NFACES * ST; tt += NP) {
         const int f = tt / ST, t = tt - f * ST;
         const ElFace rec = mesh.elface[e * NFACES + f];
@@ -562,18 +559,17 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
               for (int d = 0; d < DIM; ++d) E[d] += base * nr[k * DIM + d];
             }
-            T zL[ND], zR[ND], Fd[DIM * ND];
+            // sum_d E[d] F_d is the IR flux in the direction E
+            T zL[ND], zR[ND], Fe[ND];
 #pragma unroll
             for (int c = 0; c < ND; ++c) {
               const T zn = sm.Zn[el][f][u][c];
               zL[c] = side == 0 ? zo[c] : zn;
               zR[c] = side == 0 ? zn : zo[c];
             }
-            ir_flux_z<DIM, DIM>(phys, zL, zR, ident, Fd);
+            ir_flux_z<DIM, 1>(phys, zL, zR, E, Fe);
 #pragma unroll
-            for (int d = 0; d < DIM; ++d)
-#pragma unroll
-              for (int c = 0; c < ND; ++c) h[c] = h[c] + E[d] * Fd[d * ND + c];
+            for (int c = 0; c < ND; ++c) h[c] = h[c] + Fe[c];
           }
           if (side == 0) {
 #pragma unroll
