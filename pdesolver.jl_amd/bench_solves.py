@@ -40,6 +40,19 @@ def newton_options(pc: str = "ilu0", restart: int = 30):
                                   gmres_restart=restart)
 
 
+def retirement(inst, reps):
+    """When each instance of the batch converged (seconds from the start of
+    the batched solve), and the instance that retired last."""
+    lat = [r.elapsed_s for r in reps]
+    last = int(np.argmax(lat))
+    m, op, ph, q = inst[last]
+    return {"batched_retire_p50_s": float(np.percentile(lat, 50)),
+            "batched_retire_p99_s": float(np.percentile(lat, 99)),
+            "batched_last": f"instance {last}: {op.kind} p={op.degree} {m.numEl} elements, "
+                            f"{reps[last].iterations} Newton / {reps[last].total_krylov_iters} Krylov, "
+                            f"retired at {lat[last]:.1f} s"}
+
+
 def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_tol: float = 1e-8,
         only: str = "both", pc: str = "ilu0", restart: int = 30):
     t0 = time.perf_counter()
@@ -58,7 +71,8 @@ def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_to
         bs.free()
         return {"instances": count, "batched_s": t_batch, "batched_solves_per_s": count / t_batch,
                 "converged_batched": sum(1 for r in reps if r.converged == 1),
-                "krylov_iters_batched": int(sum(r.total_krylov_iters for r in reps))}
+                "krylov_iters_batched": int(sum(r.total_krylov_iters for r in reps)),
+                **retirement(inst, reps)}
     # one solve at a time: a context per instance, created up front
     ctxs = [euler.EulerData(m, op, ph) for (m, op, ph, q) in inst]
     seq_lat = []
@@ -115,6 +129,7 @@ def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_to
         "newton_iters_sequential": int(sum(r.iterations for r in seq_reps)),
         "krylov_iters_batched": int(sum(r.total_krylov_iters for r in reps)),
         "krylov_iters_sequential": int(sum(r.total_krylov_iters for r in seq_reps)),
+        **retirement(inst, reps),
         "max_rel_state_diff": worst,
         "states_agree": worst <= check_tol,
         "setup_s": t_gen,

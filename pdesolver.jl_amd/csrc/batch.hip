@@ -299,6 +299,8 @@ struct InstState {
   pdes_newton_report rep{};
   // GMRES state
   double tau = 0, tau_prev_norm = 0;  // implicit-Euler pseudo-time step and the norm it was last updated at
+  int nit = 0;               // Newton steps taken
+  bool in_solve = false;     // its linear system is being solved (GMRES cycles)
   bool g_iter = false;       // still iterating in the current GMRES cycle
   bool g_done = false;       // converged in GMRES
   int g_total = 0, g_j = 0;
@@ -454,56 +456,182 @@ int pc_apply_masked(pdes_batch_group* g, const double* b, double* x, const int32
   return st;
 }
 
-// Batched GMRES(restart), right preconditioned, x0 = 0, for the instances in
-// the current work list (the Newton-active ones); b and x are packed vectors.
-// The Arnoldi steps, Givens rotations, residual estimates and convergence
-// tests of every instance run on the device: each instance carries its own
-// flags (iterating in this cycle / not yet converged / took part in this
-// cycle) that the vector kernels, the BSR product and the preconditioner
-// honour, so no host round trip happens inside a cycle except a check every
-// few steps for whether any instance is still iterating.
-int batch_gmres(pdes_batch_group* g, const pdes_newton_opts* o, const double* b, double* x,
-                const std::vector<int>& chunks, const std::vector<int>& insts) {
+// ||R_i||_{M^-1} for the active instances (work list must select them); writes rep.res_norm
+int batch_residual_norms(pdes_batch_group* g, std::vector<int>& chunks, std::vector<int>& insts, bool first) {
   pdes_ctx* c = g->c;
+  int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+  if (st) return st;
+  // weighted squares: seg_dot_partial with wt uses V_0 = res
+  if ((st = seg_dots(g, c->d_res, 1, 0, nullptr, c->d_Minv, (int)chunks.size(), (int)insts.size()))) return st;
+  if ((st = fetch_seg(g))) return st;
+  std::vector<int32_t> flags(g->inst.size(), 0);
+  PDES_HIP(hipMemcpyAsync(flags.data(), c->d_flag, flags.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  for (int i : insts) {
+    InstState& s = g->inst[i];
+    if (flags[i]) {
+      s.failed = true;
+      s.active = false;
+      continue;
+    }
+    const double nrm = std::sqrt(g->h_seg[(int64_t)i * g->mstride]);
+    if (first) s.rep.res_norm0 = nrm;
+    s.rep.res_norm = nrm;
+  }
+  return PDES_OK;
+}
+
+// Newton for the instances of one group, continuously batched: each
+// instance advances its own Newton steps.  Every pass of the loop (1) starts
+// a Newton step (Jacobian, preconditioner, right-hand side) for the
+// instances that need one, (2) begins a GMRES(m) cycle for every instance
+// with a linear system in progress (V_0 = b - A x; an instance that has just
+// started has x = 0, so V_0 = b), (3) takes the Newton update of the
+// instances whose linear solve has ended there (converged or out of
+// iterations) and tests their convergence, and (4) runs the cycle's Arnoldi
+// steps for the others.  No instance waits for another's linear solve, and
+// each one's arithmetic is the same as alone: every kernel is masked or
+// listed per instance and every reduction is instance-aligned.
+int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
+  pdes_ctx* c = g->c;
+  auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  std::vector<int> chunks, insts;
+  int st;
   const int R = o->gmres_restart;
   const int64_t ldv = c->numDof;
   const int ms = g->mstride;
-  const int ni = (int)insts.size();
-  const int nch = (int)chunks.size();
-  if (ni == 0) return PDES_OK;
-  const int64_t n = active_dofs(g);
-  const unsigned gi = (unsigned)ceil_div(ni, 64);
   hipStream_t sm = c->stream;
-  int st;
-  hipLaunchKernelGGL(gmres_init, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_iter, g->d_live, g->d_part,
-                     g->d_total, g->d_gs, g->d_coef, ms);
-  // x = 0 (coef[inst][0] = 0 from gmres_init)
-  hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, b, x, g->d_coef, ms, 0, c->d_el_list,
-                     c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
-  bool first = true;
-  while (true) {
-    // V_0 = b - A x for the instances not yet converged
-    if (first) {
-      hipLaunchKernelGGL(copy_masked, dim3(grid_n(n)), dim3(256), 0, sm, b, g->d_V, c->d_el_list, c->d_el_inst,
-                         g->dpe, n, g->d_live);
-    } else {
-      if ((st = apply_op_masked(g, o->jv_mode, x, g->d_w, g->d_live))) return st;
-      hipLaunchKernelGGL(sub_masked, dim3(grid_n(n)), dim3(256), 0, sm, b, g->d_w, g->d_V, c->d_el_list,
-                         c->d_el_inst, g->dpe, n, g->d_live);
+  PDES_HIP(hipMemsetAsync(c->d_flag, 0, g->inst.size() * 4, sm));
+  for (auto& s : g->inst) {
+    s.active = true;
+    s.failed = false;
+    s.rep = pdes_newton_report{};
+    s.tau = 0;
+    s.nit = 0;
+    s.in_solve = false;
+  }
+  if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
+  if ((st = batch_residual_norms(g, chunks, insts, true))) return st;
+  for (auto& s : g->inst)
+    if (s.active && s.rep.res_norm < o->res_abstol) {
+      s.active = false;
+      s.rep.converged = 1;
+      s.rep.elapsed_s = elapsed();
     }
-    first = false;
-    if ((st = seg_dots(g, g->d_V, 1, ldv, g->d_V, nullptr, nch, ni, g->d_live))) return st;
-    hipLaunchKernelGGL(gmres_restart, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_iter, g->d_live, g->d_part,
-                       g->d_total, g->d_jdone, g->d_seg, g->d_coef, ms, g->d_g, g->d_gs, R, o->krylov_reltol,
-                       o->krylov_abstol, o->krylov_itermax);
-    // one readback per cycle: does any instance iterate?
-    PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_iter, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
-    PDES_HIP(hipStreamSynchronize(sm));
-    bool any = false;
-    for (int i : insts) any = any || g->h_flags[i];
-    if (!any) break;
-    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, g->d_V, g->d_coef, ms, 0, c->d_el_list,
-                       c->d_el_inst, g->dpe, n, g->d_iter);
+  if ((st = jacobian_setup(c, o->fp64_blocks))) return st;
+  while (true) {
+    // (1) start a Newton step where one is due
+    for (auto& s : g->inst)
+      if (s.active && !s.in_solve && s.nit >= o->itermax) {
+        s.active = false;   // out of Newton iterations, not converged
+        s.rep.elapsed_s = elapsed();
+      }
+    if ((st = set_worklist(g, [](const InstState& s) { return s.active && !s.in_solve; }, chunks, insts))) return st;
+    if (!insts.empty()) {
+      if (o->globalize_euler) {
+        // per-instance updateEuler (globalization.jl:237-262)
+        std::vector<double> taus(g->inst.size(), 1.0);
+        for (int i : insts) {
+          InstState& s = g->inst[i];
+          if (s.nit == 0)
+            s.tau = o->euler_tau * (o->scale_euler ? 1.0 / s.rep.res_norm : 1.0);
+          else
+            s.tau *= s.tau_prev_norm / s.rep.res_norm;
+          s.tau_prev_norm = s.rep.res_norm;
+        }
+        for (size_t i = 0; i < g->inst.size(); ++i)
+          if (g->inst[i].tau > 0) taus[i] = g->inst[i].tau;
+        if ((st = set_ptc_scale(c, taus.data(), (int)taus.size()))) return st;
+      }
+      for (int i : insts) {
+        g->inst[i].nit += 1;
+        g->inst[i].rep.iterations = g->inst[i].nit;
+        g->inst[i].in_solve = true;
+      }
+      if ((st = assemble(c))) return st;
+      if ((st = pc_setup(c, o->pc_kind))) return st;
+      const int64_t n = active_dofs(g);
+      // b = -R(q) (the residual of these instances is current in d_res), x = 0
+      for (int i : insts) g->h_coef[(int64_t)i * ms] = -1.0;
+      if ((st = push_coef(g))) return st;
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, c->d_res, g->d_rhs, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
+      hipLaunchKernelGGL(gmres_init, dim3((unsigned)ceil_div((int64_t)insts.size(), 64)), dim3(64), 0, sm,
+                         (int)insts.size(), g->d_ilist, g->d_iter, g->d_live, g->d_part, g->d_total, g->d_gs,
+                         g->d_coef, ms);
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, g->d_rhs, g->d_dq, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
+      PDES_HIP(hipGetLastError());
+    }
+    // (2) begin a GMRES cycle for every linear solve in progress
+    if ((st = set_worklist(g, [](const InstState& s) { return s.in_solve; }, chunks, insts))) return st;
+    if (insts.empty()) break;
+    {
+      const int64_t n = active_dofs(g);
+      if ((st = apply_op_masked(g, o->jv_mode, g->d_dq, g->d_w, g->d_live))) return st;
+      hipLaunchKernelGGL(sub_masked, dim3(grid_n(n)), dim3(256), 0, sm, g->d_rhs, g->d_w, g->d_V, c->d_el_list,
+                         c->d_el_inst, g->dpe, n, g->d_live);
+      if ((st = seg_dots(g, g->d_V, 1, ldv, g->d_V, nullptr, (int)chunks.size(), (int)insts.size(), g->d_live)))
+        return st;
+      hipLaunchKernelGGL(gmres_restart, dim3((unsigned)ceil_div((int64_t)insts.size(), 64)), dim3(64), 0, sm,
+                         (int)insts.size(), g->d_ilist, g->d_iter, g->d_live, g->d_part, g->d_total, g->d_jdone,
+                         g->d_seg, g->d_coef, ms, g->d_g, g->d_gs, R, o->krylov_reltol, o->krylov_abstol,
+                         o->krylov_itermax);
+      // V_0 /= beta for the instances that iterate in this cycle (before the
+      // Newton updates below reuse the coefficient buffer)
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, g->d_V, g->d_coef, ms, 0,
+                         c->d_el_list, c->d_el_inst, g->dpe, n, g->d_iter);
+      PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_iter, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
+      PDES_HIP(hipStreamSynchronize(sm));
+    }
+    // (3) Newton update of the instances whose linear solve has ended
+    std::vector<char> ended(g->inst.size(), 0);
+    bool any_end = false, any_iter = false;
+    for (int i : insts) {
+      if (g->h_flags[i]) any_iter = true;
+      else { ended[i] = 1; any_end = true; }
+    }
+    if (any_end) {
+      std::vector<int32_t> tot(g->inst.size(), 0);
+      PDES_HIP(hipMemcpyAsync(tot.data(), g->d_total, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
+      PDES_HIP(hipStreamSynchronize(sm));
+      std::vector<int> fin_chunks, fin;
+      if ((st = set_worklist(g, [&](const InstState& s) { return ended[&s - g->inst.data()] != 0; }, fin_chunks, fin)))
+        return st;
+      for (int i : fin) {
+        g->inst[i].rep.total_krylov_iters += tot[i];
+        g->inst[i].in_solve = false;
+        g->h_coef[(int64_t)i * ms] = 1.0;
+      }
+      const int64_t n = active_dofs(g);
+      if ((st = push_coef(g))) return st;
+      // q += dq, step norms, new residual (and its norms) of these instances
+      hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, g->d_dq, c->d_q, g->d_coef, ms, 1,
+                         c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
+      if ((st = seg_dots(g, g->d_dq, 1, 0, g->d_dq, nullptr, (int)fin_chunks.size(), (int)fin.size()))) return st;
+      if ((st = fetch_seg(g))) return st;
+      for (int i : fin) g->inst[i].rep.step_norm = std::sqrt(g->h_seg[(int64_t)i * ms]);
+      if ((st = batch_residual_norms(g, fin_chunks, fin, false))) return st;
+      for (int i : fin) {
+        InstState& s = g->inst[i];
+        if (s.failed) { s.rep.elapsed_s = elapsed(); continue; }
+        if (s.rep.res_norm < o->res_abstol || s.rep.res_norm / s.rep.res_norm0 < o->res_reltol ||
+            s.rep.step_norm <= o->step_tol) {
+          s.rep.converged = 1;
+          s.active = false;
+          s.rep.elapsed_s = elapsed();
+        }
+      }
+      if (!any_iter) continue;
+      // back to the solves still iterating in this cycle
+      if ((st = set_worklist(g, [](const InstState& s) { return s.in_solve; }, chunks, insts))) return st;
+    }
+    if (!any_iter) continue;
+    // (4) the Arnoldi steps of this cycle and the update x += M^-1 V y
+    const int64_t n = active_dofs(g);
+    const int ni = (int)insts.size(), nch = (int)chunks.size();
+    const unsigned gi = (unsigned)ceil_div(ni, 64);
     for (int j = 0; j < R; ++j) {
       double* Vj = g->d_V + (int64_t)j * ldv;
       double* Vn = g->d_V + (int64_t)(j + 1) * ldv;
@@ -535,111 +663,18 @@ int batch_gmres(pdes_batch_group* g, const pdes_newton_opts* o, const double* b,
         if (!still) break;
       }
     }
-    // x += M^-1 V y for the instances of this cycle
     hipLaunchKernelGGL(gmres_solve_y, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_part, g->d_jdone, g->d_H,
                        g->d_g, g->d_coef, ms, R);
     hipLaunchKernelGGL(seg_combine, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, R, ldv, g->d_coef, ms, g->d_w,
                        c->d_el_list, c->d_el_inst, g->dpe, n, g->d_part);
     if ((st = pc_apply_masked(g, g->d_w, g->d_z, g->d_part))) return st;
-    hipLaunchKernelGGL(add_masked, dim3(grid_n(n)), dim3(256), 0, sm, g->d_z, x, c->d_el_list, c->d_el_inst, g->dpe,
-                       n, g->d_part);
+    hipLaunchKernelGGL(add_masked, dim3(grid_n(n)), dim3(256), 0, sm, g->d_z, g->d_dq, c->d_el_list, c->d_el_inst,
+                       g->dpe, n, g->d_part);
     PDES_HIP(hipGetLastError());
   }
-  PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_total, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
-  PDES_HIP(hipStreamSynchronize(sm));
-  for (int i : insts) g->inst[i].rep.total_krylov_iters += g->h_flags[i];
-  return PDES_OK;
-}
-
-// ||R_i||_{M^-1} for the active instances (work list must select them); writes rep.res_norm
-int batch_residual_norms(pdes_batch_group* g, std::vector<int>& chunks, std::vector<int>& insts, bool first) {
-  pdes_ctx* c = g->c;
-  int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
-  if (st) return st;
-  // weighted squares: seg_dot_partial with wt uses V_0 = res
-  if ((st = seg_dots(g, c->d_res, 1, 0, nullptr, c->d_Minv, (int)chunks.size(), (int)insts.size()))) return st;
-  if ((st = fetch_seg(g))) return st;
-  std::vector<int32_t> flags(g->inst.size(), 0);
-  PDES_HIP(hipMemcpyAsync(flags.data(), c->d_flag, flags.size() * 4, hipMemcpyDeviceToHost, c->stream));
-  PDES_HIP(hipStreamSynchronize(c->stream));
-  for (int i : insts) {
-    InstState& s = g->inst[i];
-    if (flags[i]) {
-      s.failed = true;
-      s.active = false;
-      continue;
-    }
-    const double nrm = std::sqrt(g->h_seg[(int64_t)i * g->mstride]);
-    if (first) s.rep.res_norm0 = nrm;
-    s.rep.res_norm = nrm;
-  }
-  return PDES_OK;
-}
-
-int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
-  pdes_ctx* c = g->c;
-  auto t0 = std::chrono::steady_clock::now();
-  std::vector<int> chunks, insts;
-  int st;
-  PDES_HIP(hipMemsetAsync(c->d_flag, 0, g->inst.size() * 4, c->stream));
-  for (auto& s : g->inst) { s.active = true; s.failed = false; s.rep = pdes_newton_report{}; s.tau = 0; }
-  if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
-  if ((st = batch_residual_norms(g, chunks, insts, true))) return st;
+  const double el = elapsed();
   for (auto& s : g->inst)
-    if (s.active && s.rep.res_norm < o->res_abstol) { s.active = false; s.rep.converged = 1; }
-  if ((st = jacobian_setup(c, o->fp64_blocks))) return st;
-  for (int it = 1; it <= o->itermax; ++it) {
-    if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
-    if (insts.empty()) break;
-    if (o->globalize_euler) {
-      // per-instance updateEuler (globalization.jl:237-262)
-      std::vector<double> taus(g->inst.size(), 1.0);
-      for (int i : insts) {
-        InstState& s = g->inst[i];
-        if (it == 1)
-          s.tau = o->euler_tau * (o->scale_euler ? 1.0 / s.rep.res_norm : 1.0);
-        else
-          s.tau *= s.tau_prev_norm / s.rep.res_norm;
-        s.tau_prev_norm = s.rep.res_norm;
-      }
-      for (size_t i = 0; i < g->inst.size(); ++i)
-        if (g->inst[i].tau > 0) taus[i] = g->inst[i].tau;
-      if ((st = set_ptc_scale(c, taus.data(), (int)taus.size()))) return st;
-    }
-    if ((st = assemble(c))) return st;
-    if ((st = pc_setup(c, o->pc_kind))) return st;
-    const int64_t n = active_dofs(g);
-    for (int i : insts) g->h_coef[(int64_t)i * g->mstride] = -1.0;
-    if ((st = push_coef(g))) return st;
-    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, c->d_res, g->d_rhs, g->d_coef, g->mstride,
-                       0, c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
-    if ((st = batch_gmres(g, o, g->d_rhs, g->d_dq, chunks, insts))) return st;
-    // q += dq, step norms, new residual norms for the active instances (the
-    // work list is unchanged by the linear solve)
-    for (int i : insts) g->h_coef[(int64_t)i * g->mstride] = 1.0;
-    if ((st = push_coef(g))) return st;
-    hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, c->stream, g->d_dq, c->d_q, g->d_coef, g->mstride, 1,
-                       c->d_el_list, c->d_el_inst, g->dpe, n, (const int32_t*)nullptr);
-    if ((st = seg_dots(g, g->d_dq, 1, 0, g->d_dq, nullptr, (int)chunks.size(), (int)insts.size()))) return st;
-    if ((st = fetch_seg(g))) return st;
-    for (int i : insts) g->inst[i].rep.step_norm = std::sqrt(g->h_seg[(int64_t)i * g->mstride]);
-    if ((st = batch_residual_norms(g, chunks, insts, false))) return st;
-    for (int i : insts) {
-      InstState& s = g->inst[i];
-      if (s.failed) continue;
-      s.rep.iterations = it;
-      if (s.rep.res_norm < o->res_abstol || s.rep.res_norm / s.rep.res_norm0 < o->res_reltol ||
-          s.rep.step_norm <= o->step_tol) {
-        s.rep.converged = 1;
-        s.active = false;
-        // retired between Newton steps: its result is final from here on
-        s.rep.elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      }
-    }
-  }
-  const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  for (auto& s : g->inst)
-    if (s.rep.converged != 1) s.rep.elapsed_s = el;
+    if (s.rep.converged != 1 && s.rep.elapsed_s == 0) s.rep.elapsed_s = el;
   c->nlist = -1;
   c->ptc_on = false;
   return PDES_OK;
