@@ -253,6 +253,27 @@ int pdes_newton_solve(pdes_ctx* ctx, const pdes_newton_opts* opts,
 int pdes_eval_functional(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
                          double* val);
 
+/* evalFunctionalDeriv_q (src/interface2.jl:259-283, the boundary-functional
+ * derivative of src/solver/euler/boundary_functional.jl): dJdq[numDof], the
+ * derivative of the scalar functional `kind` (1..6 of pdes_eval_functional)
+ * with respect to q at the current state, in the layout of q. */
+int pdes_functional_deriv(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
+                          double* dJdq);
+
+/* applyLinearOperatorTranspose (src/linearsolvers/LinearSolvers.jl:533-541,
+ * the matrix-explicit method src/linearsolvers/lo_petscmat.jl:143-161):
+ * b = J^T x with the BSR matrix of the last pdes_assemble_jacobian. */
+int pdes_bsr_matvec_transpose(pdes_ctx* ctx, const double* x, double* b);
+
+/* calcAdjoint (src/optimization/steady_adjoint.jl:30-66): solves
+ * J^T psi = -dJ/dq at the current state with GMRES(restart), right
+ * preconditioned by pc_kind built from J^T (linearSolveTranspose).  The
+ * Jacobian is assembled here (FP32 or FP64 blocks); on return the context
+ * holds J and its preconditioner again.  psi[numDof] in the layout of q. */
+int pdes_adjoint_solve(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc, int pc_kind,
+                       int fp64_blocks, double rtol, double atol, int restart, int maxit,
+                       double* psi, int* iters, double* final_resnorm);
+
 /* Batched steady solves (no single counterpart in the reference, which runs
  * one solve per process): ninst independent problems, each described as for
  * pdes_ctx_create.  Instances that share an SBP operator and physics kind are

@@ -159,7 +159,7 @@ def functional(m, op, ph, q, bcnums):
     g = ph.gamma
     f = op.face
     dim = m.dim
-    force = np.zeros(dim)
+    force = np.zeros(dim, dtype=np.result_type(q.dtype, np.float64))   # complex q: complex-step derivatives
     ent = 0.0
     mass = 0.0
     for bc in bcnums:

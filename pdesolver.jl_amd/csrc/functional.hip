@@ -10,57 +10,193 @@
 //   MassFlowData (:404-414): rho u . n.
 // integratefunctional! sums wface_k * integrand_k over the faces of the listed
 // BC groups; the sum is taken on the host in face order (deterministic).
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "ctx.hpp"
 
 namespace pdes {
 
+// the five integrands at one face node: force x, y, z (weighted by wface),
+// entropy flux, mass flow; T is double, or dual<ND> for their derivatives
+// with respect to the face-node state
+template <int DIM, class T>
+__device__ __forceinline__ void face_integrands(const PhysConst& phys, const T* qb, const double* nrm, double w,
+                                                T* o) {
+  constexpr int ND = DIM + 2;
+  // force
+  double nn = 0.0;
+  for (int d = 0; d < DIM; ++d) nn += nrm[d] * nrm[d];
+  const double fac = 1.0 / sqrt(nn);
+  double nh[DIM];
+  T normal_momentum = 0.0;
+  for (int d = 0; d < DIM; ++d) { nh[d] = nrm[d] * fac; normal_momentum = normal_momentum + nh[d] * qb[1 + d]; }
+  T qg[ND];
+  for (int c = 0; c < ND; ++c) qg[c] = qb[c];
+  for (int d = 0; d < DIM; ++d) qg[1 + d] = qg[1 + d] - nh[d] * normal_momentum;
+  T F[ND];
+  euler_flux<DIM>(phys, qg, nrm, F);
+  o[0] = w * F[1];
+  o[1] = w * F[2];
+  o[2] = DIM == 3 ? w * F[3] : T(0.0);
+  // entropy flux
+  T unet = 0.0;
+  for (int d = 0; d < DIM; ++d) unet = unet + qb[d + 1] * nrm[d];
+  const T p = pressure<DIM>(phys, qb);
+  const T s = dlog(p) - phys.gamma * dlog(qb[0]);
+  const T U = -qb[0] * s / phys.gamma_1;
+  o[3] = w * unet * U;
+  // mass flow
+  o[4] = w * unet;
+}
+
+// q_b = R q at face node k of boundary face b
 template <int DIM, int NF, int ST>
-__global__ void functional_integrand_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __restrict__ q,
-                                            int Np, const int32_t* __restrict__ bfaces,
-                                            const int32_t* __restrict__ bidx, int nface, double* __restrict__ out) {
+__device__ __forceinline__ void face_state(const DevSBP& sbp, const double* q, int Np, int64_t e, int f, int k,
+                                           double* qb) {
   constexpr int ND = DIM + 2, NFACES = DIM + 1;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nface * NF) return;
-  const int i = t / NF, k = t - i * NF;
-  const int b = bidx[i];
-  const int64_t e = bfaces[2 * b];
-  const int f = bfaces[2 * b + 1];
-  double qb[ND];
   for (int c = 0; c < ND; ++c) qb[c] = 0.0;
   for (int s = 0; s < ST; ++s) {
     const double r = sbp.interp[s * NF + k];
     const int node = sbp.perm[s * NFACES + f];
     for (int c = 0; c < ND; ++c) qb[c] += r * q[(e * Np + node) * ND + c];
   }
-  const double* nrm = mesh.nrm_bndry + ((int64_t)b * NF + k) * DIM;
-  const double w = sbp.wface[k];
-  // force
-  double nn = 0.0;
-  for (int d = 0; d < DIM; ++d) nn += nrm[d] * nrm[d];
-  const double fac = 1.0 / sqrt(nn);
-  double nh[DIM];
-  double normal_momentum = 0.0;
-  for (int d = 0; d < DIM; ++d) { nh[d] = nrm[d] * fac; normal_momentum += nh[d] * qb[1 + d]; }
-  double qg[ND];
-  for (int c = 0; c < ND; ++c) qg[c] = qb[c];
-  for (int d = 0; d < DIM; ++d) qg[1 + d] -= nh[d] * normal_momentum;
-  double F[ND];
-  euler_flux<DIM>(phys, qg, nrm, F);
-  double* o = out + (int64_t)t * 5;
-  o[0] = w * F[1];
-  o[1] = w * F[2];
-  o[2] = DIM == 3 ? w * F[3] : 0.0;
-  // entropy flux
-  double unet = 0.0;
-  for (int d = 0; d < DIM; ++d) unet += qb[d + 1] * nrm[d];
-  const double p = pressure<DIM>(phys, qb);
-  const double s = log(p) - phys.gamma * log(qb[0]);
-  const double U = -qb[0] * s / phys.gamma_1;
-  o[3] = w * unet * U;
-  // mass flow
-  o[4] = w * unet;
+}
+
+template <int DIM, int NF, int ST>
+__global__ void functional_integrand_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __restrict__ q,
+                                            int Np, const int32_t* __restrict__ bfaces,
+                                            const int32_t* __restrict__ bidx, int nface, double* __restrict__ out) {
+  constexpr int ND = DIM + 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nface * NF) return;
+  const int i = t / NF, k = t - i * NF;
+  const int b = bidx[i];
+  double qb[ND];
+  face_state<DIM, NF, ST>(sbp, q, Np, bfaces[2 * b], bfaces[2 * b + 1], k, qb);
+  face_integrands<DIM>(phys, qb, mesh.nrm_bndry + ((int64_t)b * NF + k) * DIM, sbp.wface[k], out + (int64_t)t * 5);
+}
+
+// evalFunctionalDeriv_q (src/interface2.jl:259-283) for a scalar functional
+// sum_k wts . integrands_k: at each face node of the selected boundary faces,
+// g_k = sum_v wts[v] d(integrand_v)/dq_b (forward mode); zero elsewhere
+template <int DIM, int NF, int ST>
+__global__ void functional_face_grad_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __restrict__ q,
+                                            int Np, const int32_t* __restrict__ bfaces, int nbndry,
+                                            const int32_t* __restrict__ sel, const double* __restrict__ wts,
+                                            double* __restrict__ grad) {
+  constexpr int ND = DIM + 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbndry * NF) return;
+  const int b = t / NF, k = t - b * NF;
+  double* gk = grad + (int64_t)t * ND;
+  if (!sel[b]) {
+    for (int c = 0; c < ND; ++c) gk[c] = 0.0;
+    return;
+  }
+  double qv[ND];
+  face_state<DIM, NF, ST>(sbp, q, Np, bfaces[2 * b], bfaces[2 * b + 1], k, qv);
+  dual<ND> qb[ND], o[5];
+  for (int c = 0; c < ND; ++c) { qb[c] = dual<ND>(qv[c]); qb[c].d[c] = 1.0; }
+  face_integrands<DIM>(phys, qb, mesh.nrm_bndry + ((int64_t)b * NF + k) * DIM, sbp.wface[k], o);
+  for (int c = 0; c < ND; ++c) {
+    double g = 0.0;
+    for (int v = 0; v < 5; ++v) g += wts[v] * o[v].d[c];
+    gk[c] = g;
+  }
+}
+
+// dJ/dq at volume node i of element e: sum over the element's boundary faces f
+// that contain node i (stencil position s) of sum_k R[s][k] g_k; one thread
+// per (element, node), so every entry has one writer
+template <int DIM, int NF, int ST>
+__global__ void functional_gather_kernel(DevMesh mesh, DevSBP sbp, int Np, const double* __restrict__ grad,
+                                         double* __restrict__ dJdq) {
+  constexpr int ND = DIM + 2, NFACES = DIM + 1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)mesh.numEl * Np) return;
+  const int64_t e = t / Np;
+  const int i = (int)(t - e * Np);
+  double acc[ND];
+  for (int c = 0; c < ND; ++c) acc[c] = 0.0;
+  for (int f = 0; f < NFACES; ++f) {
+    const ElFace r = mesh.elface[e * NFACES + f];
+    if (r.x >= 0) continue;
+    const int s = sbp.nodepos[i * NFACES + f];
+    if (s < 0) continue;
+    for (int k = 0; k < NF; ++k) {
+      const double w = sbp.interp[s * NF + k];
+      const double* gk = grad + ((int64_t)r.y * NF + k) * ND;
+      for (int c = 0; c < ND; ++c) acc[c] += w * gk[c];
+    }
+  }
+  for (int c = 0; c < ND; ++c) dJdq[t * ND + c] = acc[c];
+}
+
+// dJ/dq of the scalar functional `kind` (1..6) over the BC groups bcnums,
+// into the device vector d_out [numDof]
+int functional_grad_device(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* d_out) {
+  PDES_CHECK(c && bcnums && d_out && nbc > 0 && kind >= 1 && kind <= 6, PDES_ERR_ARG,
+             "bad argument (the derivative needs a scalar functional, kind 1..6)");
+  const int nb = (int)(c->h_bndryfaces.size() / 2), Nf = c->Nf, dim = c->dim;
+  std::vector<int32_t> sel(std::max(1, nb), 0);
+  for (int i = 0; i < nbc; ++i) {
+    PDES_CHECK(bcnums[i] >= 0 && bcnums[i] < c->numBC, PDES_ERR_ARG, "BC number %d out of range", bcnums[i]);
+    for (int b = c->h_bndry_offsets[bcnums[i]]; b < c->h_bndry_offsets[bcnums[i] + 1]; ++b) sel[b] = 1;
+  }
+  // weights of the five integrands that make up the functional
+  const double aoa = c->phys.aoa, qdyn = 0.5 * c->phys.rho_free * c->phys.Ma * c->phys.Ma;
+  const int up = dim == 2 ? 1 : 2;
+  double wts[5] = {0, 0, 0, 0, 0};
+  switch (kind) {
+    case 1: case 5: wts[0] = -std::sin(aoa); wts[up] = std::cos(aoa); break;
+    case 2: case 6: wts[0] = std::cos(aoa); wts[up] = std::sin(aoa); break;
+    case 3: wts[3] = 1.0; break;
+    case 4: wts[4] = 1.0; break;
+  }
+  if (kind >= 5)
+    for (double& w : wts) w /= qdyn;
+  int32_t *d_bf = nullptr, *d_sel = nullptr;
+  double *d_w = nullptr, *d_grad = nullptr;
+  const size_t ngrad = (size_t)std::max(1, nb) * Nf * c->nd;
+  hipError_t he = hipMalloc((void**)&d_bf, std::max<size_t>(1, c->h_bndryfaces.size()) * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc((void**)&d_sel, sel.size() * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc((void**)&d_w, sizeof(wts));
+  if (he == hipSuccess) he = hipMalloc((void**)&d_grad, ngrad * sizeof(double));
+  if (he == hipSuccess && nb > 0)
+    he = hipMemcpyAsync(d_bf, c->h_bndryfaces.data(), c->h_bndryfaces.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                        c->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(d_sel, sel.data(), sel.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(d_w, wts, sizeof(wts), hipMemcpyHostToDevice, c->stream);
+  bool launched = false;
+  if (he == hipSuccess) {
+    const int gb = (int)ceil_div((int64_t)std::max(1, nb) * Nf, 128);
+    const int ge = (int)ceil_div((int64_t)c->numEl * c->Np, 128);
+#define GRAD_LAUNCH(D, NFV, STV)                                                                                   \
+    if (!launched && dim == D && Nf == NFV && c->stencil == STV) {                                                \
+      if (nb > 0)                                                                                                  \
+        hipLaunchKernelGGL((functional_face_grad_kernel<D, NFV, STV>), dim3(gb), dim3(128), 0, c->stream,          \
+                           c->devmesh(), c->devsbp(), c->pc, c->d_q, c->Np, d_bf, nb, d_sel, d_w, d_grad);        \
+      hipLaunchKernelGGL((functional_gather_kernel<D, NFV, STV>), dim3(ge), dim3(128), 0, c->stream, c->devmesh(), \
+                         c->devsbp(), c->Np, d_grad, d_out);                                                       \
+      launched = true;                                                                                             \
+    }
+    GRAD_LAUNCH(2, 2, 3) GRAD_LAUNCH(2, 3, 6) GRAD_LAUNCH(2, 4, 10) GRAD_LAUNCH(2, 5, 15)
+    GRAD_LAUNCH(2, 3, 3) GRAD_LAUNCH(2, 4, 4) GRAD_LAUNCH(2, 5, 5) GRAD_LAUNCH(2, 6, 6)
+    GRAD_LAUNCH(3, 3, 4) GRAD_LAUNCH(3, 6, 10)
+#undef GRAD_LAUNCH
+    he = hipGetLastError();
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  }
+  hipFree(d_bf);
+  hipFree(d_sel);
+  hipFree(d_w);
+  hipFree(d_grad);
+  PDES_CHECK(he == hipSuccess, PDES_ERR_HIP, "functional derivative failed: %s", hipGetErrorString(he));
+  PDES_CHECK(launched, PDES_ERR_UNSUPPORTED, "no functional kernel for this operator");
+  return PDES_OK;
 }
 
 }  // namespace pdes
@@ -68,6 +204,16 @@ __global__ void functional_integrand_kernel(DevMesh mesh, DevSBP sbp, PhysConst 
 using namespace pdes;
 
 extern "C" {
+
+// evalFunctionalDeriv_q: dJ/dq [numDof] of a scalar functional (kind 1..6)
+int pdes_functional_deriv(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* dJdq) {
+  PDES_CHECK(c && dJdq, PDES_ERR_ARG, "null argument");
+  int st = functional_grad_device(c, kind, bcnums, nbc, c->d_work);
+  if (st) return st;
+  PDES_HIP(hipMemcpyAsync(dJdq, c->d_work, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
 
 // kind: 0 force vector (dim values), 1 lift force, 2 drag force,
 //       3 entropy flux, 4 mass flow, 5 lift coefficient, 6 drag coefficient

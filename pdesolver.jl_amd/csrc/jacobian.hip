@@ -15,6 +15,7 @@ int jacobian_free(pdes_ctx* c) {
   if (j->d_color_el) hipFree(j->d_color_el);
   if (j->d_seed) hipFree(j->d_seed);
   if (j->d_jv) hipFree(j->d_jv);
+  if (j->d_tpos) hipFree(j->d_tpos);
   delete j;
   c->jac = nullptr;
   return PDES_OK;
@@ -154,6 +155,131 @@ int launch_ptc_shift(pdes_ctx* c, const double* x, double* y) {
   hipLaunchKernelGGL(ptc_shift_kernel, dim3((unsigned)std::min<int64_t>(16384, ceil_div(n, 256))), dim3(256), 0,
                      c->stream, x, y, c->d_ptc_diag, c->d_ptc_scale, c->d_el_inst, c->active_list(),
                      c->active_count(), c->Np, c->nd);
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+// y = J^T x: row r of J^T holds, at the pattern position of (r, c), the
+// transpose of the block stored at (c, r)
+template <int ND, class BT>
+__global__ void __launch_bounds__(256)
+bsr_spmv_t_kernel(int64_t nrows, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+                  const int64_t* __restrict__ tpos, const BT* __restrict__ vals, const double* __restrict__ x,
+                  double* __restrict__ y) {
+  const int64_t tt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = tt / ND;
+  const int a = (int)(tt - row * ND);
+  if (row >= nrows) return;
+  double acc = 0.0;
+  for (int64_t b = rowptr[row]; b < rowptr[row + 1]; ++b) {
+    const BT* blk = vals + tpos[b] * (ND * ND) + a;   // column a of the block at (c, r)
+    const double* xc = x + (int64_t)colind[b] * ND;
+#pragma unroll
+    for (int c = 0; c < ND; ++c) acc += (double)blk[c * ND] * xc[c];
+  }
+  y[tt] = acc;
+}
+
+// J <- J^T: the block at p and the block at its transposed position swap,
+// each transposed; one thread per pair (the lower position), so no conflicts
+template <int ND, class BT>
+__global__ void bsr_transpose_kernel(int64_t nnzb, const int64_t* __restrict__ tpos, BT* __restrict__ vals) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= nnzb) return;
+  const int64_t q = tpos[p];
+  if (q < p) return;
+  BT* A = vals + p * (ND * ND);
+  BT* B = vals + q * (ND * ND);
+  if (q == p) {
+#pragma unroll
+    for (int a = 0; a < ND; ++a)
+#pragma unroll
+      for (int c = a + 1; c < ND; ++c) {
+        const BT t = A[a * ND + c];
+        A[a * ND + c] = A[c * ND + a];
+        A[c * ND + a] = t;
+      }
+    return;
+  }
+  BT ta[ND * ND];
+#pragma unroll
+  for (int k = 0; k < ND * ND; ++k) ta[k] = A[k];
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) A[a * ND + c] = B[c * ND + a];
+#pragma unroll
+  for (int a = 0; a < ND; ++a)
+#pragma unroll
+    for (int c = 0; c < ND; ++c) B[a * ND + c] = ta[c * ND + a];
+}
+
+static int ensure_tpos(pdes_ctx* c) {
+  pdes_jacobian* j = c->jac;
+  if (j->d_tpos) return PDES_OK;
+  std::vector<int64_t> tpos((size_t)j->nnzb);
+  for (int64_t r = 0; r < j->nrows; ++r)
+    for (int64_t b = j->h_rowptr[r]; b < j->h_rowptr[r + 1]; ++b) {
+      const int64_t cc = j->h_colind[b];
+      const auto beg = j->h_colind.begin() + j->h_rowptr[cc], end = j->h_colind.begin() + j->h_rowptr[cc + 1];
+      const auto it = std::lower_bound(beg, end, (int32_t)r);
+      PDES_CHECK(it != end && *it == (int32_t)r, PDES_ERR_UNSUPPORTED, "block pattern is not structurally symmetric");
+      tpos[(size_t)b] = (int64_t)(it - j->h_colind.begin());
+    }
+  PDES_CHECK(hipMalloc((void**)&j->d_tpos, std::max<int64_t>(1, j->nnzb) * sizeof(int64_t)) == hipSuccess,
+             PDES_ERR_NOMEM, "device allocation of the transposed block positions failed");
+  PDES_HIP(hipMemcpyAsync(j->d_tpos, tpos.data(), tpos.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int launch_spmv_transpose(pdes_ctx* c, const double* x, double* y) {
+  PDES_CHECK(c->jac && c->jac->assembled, PDES_ERR_ARG, "Jacobian not assembled");
+  int st = ensure_tpos(c);
+  if (st) return st;
+  pdes_jacobian* j = c->jac;
+  const int64_t n = j->nrows * c->nd;
+  const dim3 grid((unsigned)ceil_div(n, 256));
+  if (c->nd == 4) {
+    if (j->fp64)
+      hipLaunchKernelGGL((bsr_spmv_t_kernel<4, double>), grid, dim3(256), 0, c->stream, j->nrows, j->d_rowptr,
+                         j->d_colind, j->d_tpos, (const double*)j->d_vals, x, y);
+    else
+      hipLaunchKernelGGL((bsr_spmv_t_kernel<4, float>), grid, dim3(256), 0, c->stream, j->nrows, j->d_rowptr,
+                         j->d_colind, j->d_tpos, (const float*)j->d_vals, x, y);
+  } else {
+    if (j->fp64)
+      hipLaunchKernelGGL((bsr_spmv_t_kernel<5, double>), grid, dim3(256), 0, c->stream, j->nrows, j->d_rowptr,
+                         j->d_colind, j->d_tpos, (const double*)j->d_vals, x, y);
+    else
+      hipLaunchKernelGGL((bsr_spmv_t_kernel<5, float>), grid, dim3(256), 0, c->stream, j->nrows, j->d_rowptr,
+                         j->d_colind, j->d_tpos, (const float*)j->d_vals, x, y);
+  }
+  PDES_HIP(hipGetLastError());
+  return PDES_OK;
+}
+
+int transpose_jacobian(pdes_ctx* c) {
+  PDES_CHECK(c->jac && c->jac->assembled, PDES_ERR_ARG, "Jacobian not assembled");
+  int st = ensure_tpos(c);
+  if (st) return st;
+  pdes_jacobian* j = c->jac;
+  const dim3 grid((unsigned)ceil_div(std::max<int64_t>(1, j->nnzb), 256));
+  if (c->nd == 4) {
+    if (j->fp64)
+      hipLaunchKernelGGL((bsr_transpose_kernel<4, double>), grid, dim3(256), 0, c->stream, j->nnzb, j->d_tpos,
+                         (double*)j->d_vals);
+    else
+      hipLaunchKernelGGL((bsr_transpose_kernel<4, float>), grid, dim3(256), 0, c->stream, j->nnzb, j->d_tpos,
+                         (float*)j->d_vals);
+  } else {
+    if (j->fp64)
+      hipLaunchKernelGGL((bsr_transpose_kernel<5, double>), grid, dim3(256), 0, c->stream, j->nnzb, j->d_tpos,
+                         (double*)j->d_vals);
+    else
+      hipLaunchKernelGGL((bsr_transpose_kernel<5, float>), grid, dim3(256), 0, c->stream, j->nnzb, j->d_tpos,
+                         (float*)j->d_vals);
+  }
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -396,6 +522,16 @@ int pdes_get_bsr_values(pdes_ctx* c, void* values) {
   PDES_CHECK(c && c->jac && c->jac->assembled && values, PDES_ERR_ARG, "Jacobian not assembled");
   const size_t bytes = (size_t)c->jac->nnzb * c->nd * c->nd * (c->jac->fp64 ? 8 : 4);
   PDES_HIP(hipMemcpyAsync(values, c->jac->d_vals, bytes, hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_bsr_matvec_transpose(pdes_ctx* c, const double* x, double* b) {
+  PDES_CHECK(c && x && b, PDES_ERR_ARG, "null argument");
+  PDES_HIP(hipMemcpyAsync(c->d_work, x, c->numDof * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int st = launch_spmv_transpose(c, c->d_work, c->d_work2);
+  if (st) return st;
+  PDES_HIP(hipMemcpyAsync(b, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
   return PDES_OK;
 }

@@ -21,6 +21,9 @@ struct pdes_jacobian {
   int32_t* d_color_el = nullptr;
   double* d_seed = nullptr;
   double* d_jv = nullptr;
+  // position of block (c, r) for the block at (r, c): the pattern is
+  // structurally symmetric (built on first use by a transposed product)
+  int64_t* d_tpos = nullptr;
 };
 
 namespace pdes {
@@ -31,6 +34,14 @@ int assemble(pdes_ctx* c);
 int assemble_coloring(pdes_ctx* c);
 // y = alpha J x + beta yadd (yadd may be null)
 int launch_spmv(pdes_ctx* c, const double* x, double* y, double alpha, const double* yadd, double beta);
+// y = J^T x (applyLinearOperatorTranspose), every block row
+int launch_spmv_transpose(pdes_ctx* c, const double* x, double* y);
+// J <- J^T in place (blocks swapped with their transposed positions and
+// transposed), so that the products and preconditioners act on J^T
+int transpose_jacobian(pdes_ctx* c);
+// dJ/dq [numDof] of the scalar boundary functional `kind` (1..6) over the BC
+// groups bcnums, into a device vector (functional.hip)
+int functional_grad_device(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* d_out);
 // y -= (M / tau_vec) x for the implicit-Euler globalization (no-op when off)
 int launch_ptc_shift(pdes_ctx* c, const double* x, double* y);
 // per-instance pseudo-time steps tau[0..n) (stored as 1/tau); turns the term on

@@ -955,9 +955,11 @@ int gmres_device(pdes_ctx* c, const double* b, double* x, double rtol, double at
         hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, W.z, x, 1.0, 1, n);
       PDES_HIP(hipStreamSynchronize(c->stream));
     }
+    // The Givens estimate res_est is the residual of x + M^-1 V y only when
+    // M^-1 is applied exactly; the block preconditioners apply it in FP32, so
+    // the loop returns to the top, where the true residual b - A x decides
+    // convergence (and, when it is still too large, seeds the next cycle).
     x_zero = false;
-    if (res_est <= std::max(rtol * beta0, atol)) { converged = true; beta = res_est; break; }
-    if (total >= maxit) { beta = res_est; break; }
   }
 done:
   gmres_keep(c, W);
@@ -1010,6 +1012,37 @@ int pdes_gmres(pdes_ctx* c, const double* b_host, double* x_host, double rtol, d
   int st = gmres_device(c, c->d_work, c->d_work2, rtol, atol, restart, maxit, mode, iters, final_resnorm, false);
   if (st) return st;
   PDES_HIP(hipMemcpyAsync(x_host, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+// calcAdjoint (src/optimization/steady_adjoint.jl:30-66): J^T psi = -dJ/dq
+// at the current state, with J assembled here, its blocks transposed in
+// place for the solve (so the chosen preconditioner is built from J^T, as
+// linearSolveTranspose applies the transposed preconditioner) and restored
+// afterwards, with the preconditioner rebuilt from J.
+int pdes_adjoint_solve(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, int pc_kind, int fp64_blocks,
+                       double rtol, double atol, int restart, int maxit, double* psi, int* iters,
+                       double* final_resnorm) {
+  PDES_CHECK(c && psi && restart > 0 && maxit >= 0, PDES_ERR_ARG, "bad argument");
+  const int64_t n = c->numDof;
+  int st = jacobian_setup(c, fp64_blocks);
+  if (st) return st;
+  c->ptc_on = false;
+  if ((st = assemble(c))) return st;
+  // rhs = -dJ/dq in d_work, psi in d_work2 (x0 = 0)
+  if ((st = functional_grad_device(c, kind, bcnums, nbc, c->d_work))) return st;
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, c->d_work, c->d_work, -1.0, 0, n);
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, c->d_work, c->d_work2, 0.0, 0, n);
+  PDES_HIP(hipGetLastError());
+  if ((st = transpose_jacobian(c))) return st;
+  int gst = pc_setup(c, pc_kind);
+  if (!gst) gst = gmres_device(c, c->d_work, c->d_work2, rtol, atol, restart, maxit, 0, iters, final_resnorm, true);
+  // restore J and its preconditioner whatever the solve returned
+  if ((st = transpose_jacobian(c))) return st;
+  if ((st = pc_setup(c, pc_kind))) return st;
+  if (gst) return gst;
+  PDES_HIP(hipMemcpyAsync(psi, c->d_work2, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
   return PDES_OK;
 }

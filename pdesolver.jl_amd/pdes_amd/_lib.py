@@ -54,6 +54,10 @@ _SIGS = {
                    _i32p, _dp],
     "pdes_newton_solve": [_vp, C.POINTER(abi.NewtonOpts), C.POINTER(abi.NewtonReport)],
     "pdes_eval_functional": [_vp, C.c_int, _vp, C.c_int, _dp],
+    "pdes_functional_deriv": [_vp, C.c_int, _vp, C.c_int, _vp],
+    "pdes_bsr_matvec_transpose": [_vp, _vp, _vp],
+    "pdes_adjoint_solve": [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
+                           C.c_int, _vp, _i32p, _dp],
     "pdes_batch_create": [C.c_int, _vp, _vp, _vp, C.POINTER(_vp)],
     "pdes_batch_destroy": [_vp],
     "pdes_batch_num_groups": [_vp, C.POINTER(C.c_int)],

@@ -88,6 +88,46 @@ class HIPBSRLO(AbstractLO):
         b[...] = out.reshape(b.shape)
 
 
+def applyLinearOperatorTranspose(eqn: EulerData, x, b):
+    """applyLinearOperatorTranspose: b = J^T x with the assembled BSR Jacobian."""
+    xc = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(xc)
+    check(lib().pdes_bsr_matvec_transpose(eqn.ctx, xc.ctypes.data, out.ctypes.data))
+    b[...] = out.reshape(b.shape)
+
+
+FUNCTIONALS = {"lift": 1, "drag": 2, "entropyflux": 3, "massflow": 4, "liftCoefficient": 5, "dragCoefficient": 6}
+
+
+def _bcs(bcnums):
+    return np.ascontiguousarray(bcnums, dtype=np.int32)
+
+
+def evalFunctionalDeriv_q(eqn: EulerData, functional: str, bcnums):
+    """evalFunctionalDeriv_q (interface2.jl:259-283): dJ/dq in the layout of q
+    for a scalar boundary functional over the (0-based) BC groups bcnums."""
+    bc = _bcs(bcnums)
+    out = np.empty(eqn.q.shape, dtype=np.float64)
+    check(lib().pdes_functional_deriv(eqn.ctx, FUNCTIONALS[functional], bc.ctypes.data, len(bc), out.ctypes.data))
+    return out
+
+
+def calcAdjoint(mesh, sbp, eqn: EulerData, functional: str, bcnums, opts: "NewtonOptions | None" = None,
+                rtol: float = 1e-10):
+    """calcAdjoint (steady_adjoint.jl:30-66): psi with J^T psi = -dJ/dq at the
+    current state; the preconditioner (opts.pc_kind) is built from J^T.
+    Returns (psi, GMRES iterations, final residual norm)."""
+    opts = opts or NewtonOptions()
+    bc = _bcs(bcnums)
+    psi = np.empty(eqn.q.shape, dtype=np.float64)
+    its = C.c_int32()
+    res = C.c_double()
+    check(lib().pdes_adjoint_solve(eqn.ctx, FUNCTIONALS[functional], bc.ctypes.data, len(bc), opts.pc_kind,
+                                   int(opts.fp64_blocks), rtol, 0.0, opts.gmres_restart, opts.krylov_itermax,
+                                   psi.ctypes.data, C.byref(its), C.byref(res)))
+    return psi, its.value, res.value
+
+
 class HIPMatFreeLO(AbstractLO):
     """Matrix-free operator: the directional derivative of the residual
     (complex-step product of the reference) evaluated on the device."""
