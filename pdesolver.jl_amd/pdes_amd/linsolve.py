@@ -96,6 +96,22 @@ def applyLinearOperatorTranspose(eqn: EulerData, x, b):
     b[...] = out.reshape(b.shape)
 
 
+def evalResidual_revq(mesh, sbp, eqn: EulerData, opts, input_array, output_array, t: float = 0.0,
+                      zero_output: bool = True):
+    """evalResidual_revq (interface2.jl:392-428): output += psi^T dR/dq at the
+    device state, psi = input_array (overwrite when zero_output).  The product
+    is (dR/dq)^T psi with the FP64-block Jacobian assembled here at that state,
+    which replaces the context's assembled Jacobian."""
+    evalJacobian(mesh, sbp, eqn, opts, fp64_blocks=True)
+    out = np.empty(eqn.q.shape, dtype=np.float64)
+    applyLinearOperatorTranspose(eqn, np.reshape(input_array, eqn.q.shape), out)
+    dst = np.reshape(output_array, eqn.q.shape)
+    if zero_output:
+        dst[...] = out
+    else:
+        dst += out
+
+
 FUNCTIONALS = {"lift": 1, "drag": 2, "entropyflux": 3, "massflow": 4, "liftCoefficient": 5, "dragCoefficient": 6}
 
 

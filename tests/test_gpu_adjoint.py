@@ -1,12 +1,14 @@
 """Adjoint of the steady residual for boundary functionals, through the C ABI:
 the functional derivative dJ/dq (evalFunctionalDeriv_q), the transposed
 Jacobian product (applyLinearOperatorTranspose) and the adjoint solve
-J^T psi = -dJ/dq (calcAdjoint)."""
+J^T psi = -dJ/dq (calcAdjoint), and the reverse-mode product psi^T dR/dq
+(evalResidual_revq)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
 from pdes_amd import abi, euler, linsolve, problems
+from helpers import setup, state_for
 
 pytestmark = pytest.mark.gpu
 
@@ -110,4 +112,31 @@ def test_adjoint_matches_dense_solve(kind, n, p, opk, pc):
     assert np.abs(r).max() <= 1e-10 * np.abs(g).max(), np.abs(r).max()
     ref = np.linalg.solve(J.T.toarray(), -g.ravel()).reshape(q.shape)
     assert np.abs(psi - ref).max() <= 1e-5 * np.abs(ref).max()
+    eqn.free()
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", [("vortex", 3, 2, "omega", "roe"), ("vortex", 3, 2, "diage", "ir"),
+                                               ("vortex", 3, 2, "omega", "es"), ("cube", 1, 1, "omega", "roe")])
+def test_residual_revq_matches_complex_step(oracle, kind, n, p, opk, flux):
+    """psi^T dR/dq (evalResidual_revq) against the oracle's complex-step
+    directional derivatives: (psi^T dR/dq) . v = psi . (dR/dq v) for several
+    v; with zero_output=False the product is added to the output."""
+    print(f"case {kind} n={n} p={p} {opk} {flux}", flush=True)
+    m, op, ph, b = setup(kind, n, p, opk, flux, Ma=0.5, aoa_deg=2.0)
+    q = state_for(kind, m, ph)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    rng = np.random.default_rng(11)
+    psi = rng.standard_normal(q.size)
+    qbar = np.zeros(q.size)
+    linsolve.evalResidual_revq(m, op, eqn, None, psi, qbar)
+    for k in range(3):
+        v = rng.standard_normal(q.shape)
+        jv = oracle.jv_complex_step(b, q, v)
+        a, c = float(psi @ jv.ravel()), float(qbar @ v.ravel())
+        scale = float(np.abs(psi).sum() * np.abs(jv).max())
+        assert abs(a - c) <= 1e-12 * scale, (k, a, c, scale)
+    acc = np.ones(q.size)
+    linsolve.evalResidual_revq(m, op, eqn, None, psi, acc, zero_output=False)
+    assert np.array_equal(acc, np.ones(q.size) + qbar)
     eqn.free()
