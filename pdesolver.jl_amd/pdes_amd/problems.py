@@ -125,6 +125,23 @@ def make_problem(kind: str, n: int, p: int, op_kind: str = "omega", flux: str = 
     return m, op, phys
 
 
+def smb_vortex_problem(path: str, p: int, op_kind: str = "omega", flux: str = "roe",
+                       bc_tags=None, vortex_Min: float = 0.95):
+    """Steady isentropic vortex on a serial PUMI mesh file, isentropicVortexBC
+    on every boundary group (the convergence cases of
+    test/euler/convergence/p*/conservative_dg, BC1 => [0, 1, 2, 3]).
+    Returns (mesh, op, phys)."""
+    from . import pumi
+    op = sbpmod.get_operator(2, p, op_kind)
+    m = pumi.load_smb(path, op, bc_tags=bc_tags)
+    bc = [PDES_BC_ISENTROPIC_VORTEX] * m.numBC
+    if flux != "roe":
+        raise ValueError("the convergence cases use the Roe face flux")
+    phys = Physics(Ma=0.5, aoa_deg=0.0, vortex_Min=vortex_Min, volume_integral_type=1,
+                   face_flux=PDES_FLUX_ROE, bc_type=bc)
+    return m, op, phys
+
+
 def config3_instances(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, amp: float = 1e-3):
     """Batched uneven solves: `count` 2D vortex problems with the grid edge
     count uniform in nmin..nmax, operator Omega or diagonal-E and inflow Mach
