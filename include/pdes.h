@@ -246,6 +246,31 @@ typedef struct {
 int pdes_newton_solve(pdes_ctx* ctx, const pdes_newton_opts* opts,
                       pdes_newton_report* report);
 
+/* evalHomotopy with homotopy_function "FirstOrderDissipation"
+ * (src/solver/euler/homotopy.jl:11-31 -> calcHomotopyDiss :51-249, without
+ * the boundary term, homotopy_addBoundaryIntegrals = false): res[numDof] =
+ * D(q) at the device state.  Single (unbatched) contexts, Np <= 64. */
+int pdes_eval_homotopy(pdes_ctx* ctx, double* res);
+
+/* b = (dD/dq) v at the device state (the homotopy term of the complex-step
+ * product of HomotopyPetscMatFreeLO, predictor_corrector.jl). */
+int pdes_homotopy_jv(pdes_ctx* ctx, const double* v, double* b);
+
+/* Jacobian of the homotopy system (1 - lambda) R + lambda D into the BSR
+ * matrix (HomotopyMatLO calcLinearOperator, predictor_corrector.jl: the
+ * physics Jacobian scaled by 1 - lambda plus lambda dD/dq). */
+int pdes_assemble_homotopy_jacobian(pdes_ctx* ctx, double lambda);
+
+/* predictorCorrectorHomotopy (src/NonlinearSolvers/predictor_corrector.jl:
+ * 226-326) on the device state: lambda from 1 to 0, Newton corrector on
+ * (1 - lambda) R + lambda D with the options' PC and Krylov settings, tangent
+ * predictor with step control by psi_max (opts "homotopy_psi_max", default
+ * 10 degrees) and the corrector step length.  report.iterations counts
+ * predictor iterations; converged when ||R|| <= res_abstol or
+ * ||R||/||R_0|| <= res_reltol within itermax. */
+int pdes_homotopy_solve(pdes_ctx* ctx, const pdes_newton_opts* opts, double psi_max,
+                        pdes_newton_report* report);
+
 /* Boundary force functionals (calcBndryFunctional,
  * src/solver/euler/boundary_functional.jl:82-126):
  * kind 0 = force vector (dim values), 1 = lift, 2 = drag, 3 = entropy flux,

@@ -18,6 +18,7 @@
 //   checkDensity / checkPressure  src/solver/euler/euler.jl:543-611
 //   calcJacobianSparse (complex step, distance-2 coloring)  src/jacobian/jacobian.jl:365-446
 //   SparseMatrixCSC(mesh, Tv, INVISCID, face_type)           src/jacobian/sparse.jl:250-495
+//   calcHomotopyDiss (homotopy_addBoundaryIntegrals = false)  src/solver/euler/homotopy.jl:51-249
 // The SBP application routines (weakdifferentiate!, interiorfaceinterpolate!,
 // interiorfaceintegrate!, boundaryinterpolate!, boundaryintegrate!) live in
 // the external SummationByParts.jl; they are restated from their documented
@@ -275,6 +276,84 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
 }
 
 // element adjacency through interfaces
+// calcHomotopyDiss (src/solver/euler/homotopy.jl:51-249), the first-order
+// dissipation of the homotopy globalisation, without its boundary term (the
+// default homotopy_addBoundaryIntegrals = false, src/input/read_input.jl:438):
+//   volume, per reference direction d: t1 = H^-1 Q_d q (differentiateElement!),
+//     t2_j = lambda_max(q_j, dxidx[d, :, j]) t1_j, res += Q_d^T t2
+//     (weakDifferentiateElement!, trans = true);
+//   interfaces: flux_k = lambda_max(avg(qL_k, qR_k), nrm_k) (qL_k - qR_k) / 2 at
+//     each face node (getLambdaMaxSimple, IR_stab.jl:310-325), resL += R_L^T B
+//     flux, resR -= R_R^T B flux (interiorFaceIntegrate!, Add);
+//   then res = -res.
+template <class T>
+void homotopy_diss(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph, const T* q,
+                   T* res) {
+  const int dim = m->dim, nd = dim + 2, Np = s->Np, Nf = s->Nf, nfaces = dim + 1;
+  Params p = make_params(ph, dim);
+  const size_t n = (size_t)m->numEl * Np * nd;
+  for (size_t i = 0; i < n; ++i) res[i] = 0.0;
+  auto Q = [&](int d, int i, int j) { return s->Q[((size_t)d * Np + i) * Np + j]; };
+  std::vector<T> t2((size_t)Np * nd);
+  for (int e = 0; e < m->numEl; ++e) {
+    const T* qe = q + (size_t)e * Np * nd;
+    T* re_ = res + (size_t)e * Np * nd;
+    for (int d1 = 0; d1 < dim; ++d1) {
+      for (int j = 0; j < Np; ++j) {
+        double dir[3];
+        for (int x = 0; x < dim; ++x) dir[x] = m->dxidx[(((size_t)e * Np + j) * dim + x) * dim + d1];
+        const T lam = getLambdaMax(p, qe + (size_t)j * nd, dir);
+        for (int k = 0; k < nd; ++k) {
+          T t1 = 0.0;
+          for (int jj = 0; jj < Np; ++jj) t1 += Q(d1, j, jj) * qe[(size_t)jj * nd + k];
+          t2[(size_t)j * nd + k] = lam * (t1 / s->w[j]);
+        }
+      }
+      for (int i = 0; i < Np; ++i)
+        for (int j = 0; j < Np; ++j) {
+          const double qji = Q(d1, j, i);
+          for (int k = 0; k < nd; ++k) re_[(size_t)i * nd + k] += qji * t2[(size_t)j * nd + k];
+        }
+    }
+  }
+  std::vector<T> qL((size_t)Nf * nd), qR((size_t)Nf * nd), ff((size_t)Nf * nd);
+  for (int i = 0; i < m->numInterfaces; ++i) {
+    const int* it = m->interfaces + 5 * i;
+    const int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
+    for (int kf = 0; kf < Nf; ++kf) {
+      const int kR = s->nbrperm[kf * s->norient + o];
+      for (int k = 0; k < nd; ++k) {
+        T vl = 0.0, vr = 0.0;
+        for (int st = 0; st < s->stencil; ++st) {
+          vl += s->interp[st * Nf + kf] * q[((size_t)eL * Np + s->perm[st * nfaces + fL]) * nd + k];
+          vr += s->interp[st * Nf + kR] * q[((size_t)eR * Np + s->perm[st * nfaces + fR]) * nd + k];
+        }
+        qL[kf * nd + k] = vl;
+        qR[kf * nd + k] = vr;
+      }
+    }
+    for (int kf = 0; kf < Nf; ++kf) {
+      T qa[5];
+      for (int k = 0; k < nd; ++k) qa[k] = 0.5 * (qL[kf * nd + k] + qR[kf * nd + k]);
+      const T lam = getLambdaMax(p, qa, m->nrm_face + ((size_t)i * Nf + kf) * dim);
+      for (int k = 0; k < nd; ++k) ff[kf * nd + k] = 0.5 * lam * (qL[kf * nd + k] - qR[kf * nd + k]);
+    }
+    for (int st = 0; st < s->stencil; ++st) {
+      const int nodeL = s->perm[st * nfaces + fL], nodeR = s->perm[st * nfaces + fR];
+      for (int kf = 0; kf < Nf; ++kf) {
+        const int kR = s->nbrperm[kf * s->norient + o];
+        const double facL = s->interp[st * Nf + kf] * s->wface[kf];
+        const double facR = s->interp[st * Nf + kR] * s->wface[kR];
+        for (int k = 0; k < nd; ++k) {
+          res[((size_t)eL * Np + nodeL) * nd + k] += facL * ff[kf * nd + k];
+          res[((size_t)eR * Np + nodeR) * nd + k] -= facR * ff[kf * nd + k];
+        }
+      }
+    }
+  }
+  for (size_t i = 0; i < n; ++i) res[i] = -res[i];
+}
+
 std::vector<std::vector<int>> element_neighbors(const pdes_mesh_desc* m) {
   std::vector<std::vector<int>> nb(m->numEl);
   for (int i = 0; i < m->numInterfaces; ++i) {
@@ -318,6 +397,23 @@ int oracle_jv_complex_step(const pdes_mesh_desc* m, const pdes_sbp_desc* s, cons
   int st = residual<cplx>(m, s, ph, qc.data(), r.data(), false);
   for (size_t i = 0; i < n; ++i) b[i] = r[i].imag() / h;
   return st;
+}
+
+int oracle_homotopy(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph, const double* q,
+                    double* res) {
+  homotopy_diss<double>(m, s, ph, q, res);
+  return PDES_OK;
+}
+
+// directional derivative of calcHomotopyDiss by the complex step
+int oracle_homotopy_jv(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
+                       const double* q, const double* v, double h, double* b) {
+  size_t n = (size_t)m->numEl * s->Np * (m->dim + 2);
+  std::vector<cplx> qc(n), r(n);
+  for (size_t i = 0; i < n; ++i) qc[i] = cplx(q[i], h * v[i]);
+  homotopy_diss<cplx>(m, s, ph, qc.data(), r.data());
+  for (size_t i = 0; i < n; ++i) b[i] = r[i].imag() / h;
+  return PDES_OK;
 }
 
 // Block sparsity of SparseMatrixCSC(mesh, Tv, INVISCID, face_type) (sparse.jl:250-495),

@@ -302,7 +302,7 @@ int pdes_ctx_destroy(pdes_ctx* c) {
                   c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_Minv_node, c->d_q,
                   c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
                   c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale,
-                  c->d_inst_active, c->d_qbc};
+                  c->d_inst_active, c->d_qbc, c->d_w};
   for (void* p : ptrs)
     if (p) hipFree(p);
   if (c->own_stream && c->stream) hipStreamDestroy(c->stream);

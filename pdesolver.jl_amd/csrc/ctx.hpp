@@ -55,6 +55,10 @@ struct pdes_ctx {
   double* d_ptc_diag = nullptr;
   double* d_ptc_scale = nullptr;
   bool ptc_on = false;
+  // homotopy (predictorCorrectorHomotopy): when hom_lambda >= 0 the Newton
+  // system is H = (1 - lambda) R + lambda D with D the first-order dissipation
+  double hom_lambda = -1.0;
+  double* d_w = nullptr;      // SBP weights H (uploaded on first homotopy use)
   // packed batches: per-instance 0/1 mask honoured by the BSR product and the
   // preconditioner apply (instances with 0 are skipped); null = all
   const int32_t* d_mask = nullptr;

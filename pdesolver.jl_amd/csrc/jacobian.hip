@@ -339,7 +339,7 @@ template <int ND, class BT>
 __global__ void coloring_scatter_kernel(const int32_t* __restrict__ els, int64_t n, int np, int nfaces,
                                         const ElFace* __restrict__ elface, const int64_t* __restrict__ rowptr,
                                         const int32_t* __restrict__ colind, BT* __restrict__ vals,
-                                        const double* __restrict__ jv, int j, int a) {
+                                        const double* __restrict__ jv, int j, int a, bool add, double scale) {
   // thread -> (element slot, row element r in {e, face neighbours}, row node, row dof)
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = (int64_t)(nfaces + 1) * np * ND;
@@ -354,7 +354,15 @@ __global__ void coloring_scatter_kernel(const int32_t* __restrict__ els, int64_t
   if (re < 0) return;
   const int64_t row = re * np + i;
   const int64_t pos = bsr_find(rowptr, colind, row, (int32_t)(e * np + j));
-  if (pos >= 0) vals[pos * ND * ND + rr * ND + a] = (BT)jv[row * ND + rr];
+  if (pos < 0) return;
+  BT* dst = vals + pos * ND * ND + rr * ND + a;
+  *dst = add ? (BT)((double)*dst + scale * jv[row * ND + rr]) : (BT)jv[row * ND + rr];
+}
+
+template <class BT>
+__global__ void bsr_scale_kernel(int64_t n, BT* __restrict__ vals, double s) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) vals[t] = (BT)(s * (double)vals[t]);
 }
 
 // -M/tau_vec on the diagonal blocks (the implicit-Euler term of applyEuler)
@@ -415,8 +423,10 @@ static int coloring_setup(pdes_ctx* c) {
   return PDES_OK;
 }
 
+// hom: differentiate the homotopy dissipation instead of the residual and add
+// scale * its derivative to the stored values (no implicit-Euler term)
 template <int ND, class BT>
-static int coloring_run(pdes_ctx* c) {
+static int coloring_run(pdes_ctx* c, bool hom = false, double scale = 1.0) {
   pdes_jacobian* J = c->jac;
   const int np = c->Np, nf = c->nfaces;
   // the residual runs over the whole mesh (the work list is not used here)
@@ -430,19 +440,20 @@ static int coloring_run(pdes_ctx* c) {
       for (int a = 0; a < ND && !st; ++a) {
         hipLaunchKernelGGL(seed_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, els, n, np, ND, j,
                            a, 1.0, J->d_seed);
-        st = launch_residual(c, c->d_q, J->d_seed, nullptr, J->d_jv, false);
+        st = hom ? launch_homotopy(c, c->d_q, J->d_seed, 0.0, 1.0, nullptr, J->d_jv)
+                 : launch_residual(c, c->d_q, J->d_seed, nullptr, J->d_jv, false);
         if (st) break;
         const int64_t nt = n * (nf + 1) * np * ND;
         hipLaunchKernelGGL((coloring_scatter_kernel<ND, BT>), dim3((unsigned)ceil_div(nt, 256)), dim3(256), 0,
                            c->stream, els, n, np, nf, c->d_elface, J->d_rowptr, J->d_colind, (BT*)J->d_vals, J->d_jv,
-                           j, a);
+                           j, a, hom, scale);
         hipLaunchKernelGGL(seed_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, c->stream, els, n, np, ND, j,
                            a, 0.0, J->d_seed);
       }
   }
   c->nlist = nlist_saved;
   if (st) return st;
-  if (c->ptc_on) {
+  if (c->ptc_on && !hom) {
     const int64_t nrows = (int64_t)c->numEl * np;
     hipLaunchKernelGGL((ptc_diag_add_kernel<ND, BT>), dim3((unsigned)ceil_div(nrows, 256)), dim3(256), 0, c->stream,
                        nrows, J->d_rowptr, J->d_colind, (BT*)J->d_vals, c->d_ptc_diag, c->d_ptc_scale, c->d_el_inst,
@@ -463,6 +474,34 @@ int assemble_coloring(pdes_ctx* c) {
   else st = c->jac->fp64 ? coloring_run<5, double>(c) : coloring_run<5, float>(c);
   if (st) return st;
   c->jac->assembled = true;
+  return PDES_OK;
+}
+
+// the Jacobian of predictorCorrectorHomotopy's system (1 - lambda) R + lambda D
+// (HomotopyMatLO calcLinearOperator, src/NonlinearSolvers/predictor_corrector.jl:
+// the physics Jacobian scaled by 1 - lambda, then lambda dD/dq added), with
+// dD/dq by colouring over the dual-number dissipation
+int assemble_homotopy(pdes_ctx* c, double lambda) {
+  PDES_CHECK(c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  PDES_CHECK(lambda >= 0.0 && lambda <= 1.0, PDES_ERR_ARG, "homotopy parameter outside [0, 1]");
+  int st = launch_assemble(c);
+  if (st) return st;
+  pdes_jacobian* J = c->jac;
+  const int64_t nv = J->nnzb * c->nd * c->nd;
+  if (lambda != 0.0) {
+    if (J->fp64)
+      hipLaunchKernelGGL(bsr_scale_kernel<double>, dim3((unsigned)ceil_div(nv, 256)), dim3(256), 0, c->stream, nv,
+                         (double*)J->d_vals, 1.0 - lambda);
+    else
+      hipLaunchKernelGGL(bsr_scale_kernel<float>, dim3((unsigned)ceil_div(nv, 256)), dim3(256), 0, c->stream, nv,
+                         (float*)J->d_vals, 1.0 - lambda);
+    PDES_HIP(hipGetLastError());
+    if ((st = coloring_setup(c))) return st;
+    if (c->nd == 4) st = J->fp64 ? coloring_run<4, double>(c, true, lambda) : coloring_run<4, float>(c, true, lambda);
+    else st = J->fp64 ? coloring_run<5, double>(c, true, lambda) : coloring_run<5, float>(c, true, lambda);
+    if (st) return st;
+  }
+  J->assembled = true;
   return PDES_OK;
 }
 
@@ -488,6 +527,14 @@ int pdes_jacobian_setup(pdes_ctx* c, int fp64_blocks) {
 int pdes_assemble_jacobian_coloring(pdes_ctx* c) {
   PDES_CHECK(c, PDES_ERR_ARG, "null context");
   int st = assemble_coloring(c);
+  if (st) return st;
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_assemble_homotopy_jacobian(pdes_ctx* c, double lambda) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  int st = assemble_homotopy(c, lambda);
   if (st) return st;
   PDES_HIP(hipStreamSynchronize(c->stream));
   return PDES_OK;

@@ -42,6 +42,12 @@ int transpose_jacobian(pdes_ctx* c);
 // dJ/dq [numDof] of the scalar boundary functional `kind` (1..6) over the BC
 // groups bcnums, into a device vector (functional.hip)
 int functional_grad_device(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* d_out);
+// out = a out + b D(q) with D the homotopy dissipation (homotopy.hip); with v
+// set, jv = a jv + b (dD/dq) v and out (when not null) takes the value part
+int launch_homotopy(pdes_ctx* c, const double* q, const double* v, double a, double b, double* out, double* jv);
+// BSR values of (1 - lambda) dR/dq + lambda dD/dq (element-wise dR/dq, dD/dq
+// by colouring)
+int assemble_homotopy(pdes_ctx* c, double lambda);
 // y -= (M / tau_vec) x for the implicit-Euler globalization (no-op when off)
 int launch_ptc_shift(pdes_ctx* c, const double* x, double* y);
 // per-instance pseudo-time steps tau[0..n) (stored as 1/tau); turns the term on

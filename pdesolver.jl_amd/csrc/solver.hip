@@ -864,7 +864,9 @@ static int norm2_dev(pdes_ctx* c, GmresWork& W, const double* x, int off) {
 
 static int apply_op(pdes_ctx* c, int mode, const double* x, double* y) {
   if (mode == 0) return launch_spmv(c, x, y, 1.0, nullptr, 0.0);
-  const int st = launch_residual(c, c->d_q, x, nullptr, y, false);
+  int st = launch_residual(c, c->d_q, x, nullptr, y, false);
+  if (!st && c->hom_lambda >= 0.0)
+    st = launch_homotopy(c, c->d_q, x, 1.0 - c->hom_lambda, c->hom_lambda, nullptr, y);
   return st ? st : launch_ptc_shift(c, x, y);
 }
 
@@ -1061,6 +1063,8 @@ int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report
              hipMalloc((void**)&rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "Newton workspace");
   auto norm_res = [&](double* out) -> int {
     int s2 = launch_residual(c, c->d_q, nullptr, res, nullptr, c->phys.check_positivity != 0);
+    if (!s2 && c->hom_lambda >= 0.0)
+      s2 = launch_homotopy(c, c->d_q, nullptr, 1.0 - c->hom_lambda, c->hom_lambda, res, nullptr);
     if (s2) return s2;
     s2 = launch_wnorm2(c, res, c->d_Minv, n, c->d_scalar);
     if (s2) return s2;
@@ -1091,7 +1095,7 @@ int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report
       if ((st = set_ptc_scale(c, &tau, 1)) != 0) goto fin;
     }
     // doRecalculation: Jacobian and preconditioner every iteration (recalc_freq 1)
-    if ((st = assemble(c)) != 0) goto fin;
+    if ((st = c->hom_lambda >= 0.0 ? assemble_homotopy(c, c->hom_lambda) : assemble(c)) != 0) goto fin;
     if ((st = pdes_pc_setup(c, o->pc_kind)) != 0) goto fin;
     hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, res, rhs, -1.0, 0, n);
     int kits = 0;
@@ -1119,6 +1123,129 @@ fin:
   c->ptc_on = false;
   hipFree(dq);
   hipFree(rhs);
+  rep->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+// predictorCorrectorHomotopy (src/NonlinearSolvers/predictor_corrector.jl:
+// HomotopyData :63-148, the loop :226-326, updateTolerances :381-440,
+// calcTangentVector :454-476, calcdHdLambda :495-530, takePredictorStep
+// :535-595).  H(q, lambda) = (1 - lambda) R(q) + lambda D(q), lambda from 1 to
+// 0: each predictor iteration runs a Newton corrector on H (tolerance
+// homotopy_tol = 1e-2 relative while lambda > 0, res_reltol at lambda = 0,
+// at most 30 iterations), then solves dH/dq t = dH/dlambda = D - R with the
+// corrector's last Jacobian and preconditioner, normalises t with the SBP
+// norm, sets the step h from the angle between successive tangents (limit
+// psi_max) and the corrector's step length (limit 1), and takes
+// lambda -= h, q += h t.  The loop ends on the physics residual:
+// ||R|| <= res_abstol or ||R|| / ||R_0|| <= res_reltol, or itermax
+// predictor iterations.
+int pdes_homotopy_solve(pdes_ctx* c, const pdes_newton_opts* o, double psi_max, pdes_newton_report* rep) {
+  PDES_CHECK(c && o && rep, PDES_ERR_ARG, "null argument");
+  PDES_CHECK(psi_max > 0, PDES_ERR_ARG, "psi_max must be positive");
+  PDES_CHECK(!o->globalize_euler, PDES_ERR_UNSUPPORTED, "homotopy with implicit-Euler globalization is not available");
+  auto t0 = std::chrono::steady_clock::now();
+  *rep = pdes_newton_report{};
+  const int64_t n = c->numDof;
+  const double eps = 2.220446049250313e-16;
+  std::vector<double> M(n), t(n), t1(n), q(n), q0(n), rhs(n);
+  double* d_t = nullptr;
+  double* d_rhs = nullptr;
+  int st = PDES_OK;
+  PDES_CHECK(hipMalloc((void**)&d_t, n * sizeof(double)) == hipSuccess &&
+             hipMalloc((void**)&d_rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "homotopy workspace");
+  auto phys_norm = [&](double* out) -> int {
+    c->hom_lambda = -1.0;
+    int s2 = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
+    if (!s2) s2 = launch_wnorm2(c, c->d_res, c->d_Minv, n, c->d_scalar);
+    if (s2) return s2;
+    double v = 0;
+    PDES_HIP(hipMemcpyAsync(&v, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));
+    if ((s2 = sync_check_flag(c))) return s2;
+    *out = std::sqrt(v);
+    return PDES_OK;
+  };
+  // calcNorm (Utils.jl:427-449) and calcL2InnerProduct (:476-493): M-weighted
+  auto mdot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+    double acc = 0.0;
+    for (int64_t i = 0; i < n; ++i) acc += a[i] * M[i] * b[i];
+    return acc;
+  };
+  double lambda = 1.0, h = 0.05, h_max = 0.1, delta_max = 1.0, hom_tol = 1e-2, tan_norm_1 = 0.0;
+  double res_norm = 0.0;
+  int iter = 1;
+  lambda -= h;   // the Jacobian at lambda = 1 is ill-conditioned; start one step in
+  {
+    std::vector<double> minv(n);
+    PDES_HIP(hipMemcpyAsync(minv.data(), c->d_Minv, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; ++i) M[i] = 1.0 / minv[i];
+  }
+  if ((st = phys_norm(&res_norm)) != 0) goto done;
+  rep->res_norm0 = res_norm;
+  while (res_norm > rep->res_norm0 * o->res_reltol && res_norm > o->res_abstol && iter < o->itermax) {
+    // updateTolerances
+    if (std::fabs(lambda) <= eps) hom_tol = o->res_reltol;
+    // corrector
+    PDES_HIP(hipMemcpyAsync(q0.data(), c->d_q, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    c->hom_lambda = lambda;
+    pdes_newton_opts no = *o;
+    no.res_reltol = hom_tol;
+    no.itermax = 30;
+    pdes_newton_report nrep{};
+    if ((st = pdes_newton_solve(c, &no, &nrep)) != 0) goto done;
+    rep->total_krylov_iters += nrep.total_krylov_iters;
+    if (std::fabs(lambda) > eps) {
+      // tangent: dH/dq t = D(q) - R(q), with the corrector's last operator
+      if ((st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false)) != 0) goto done;
+      if ((st = launch_homotopy(c, c->d_q, nullptr, 0.0, 1.0, d_rhs, nullptr)) != 0) goto done;
+      hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, c->stream, c->d_res, d_rhs, -1.0, 1, n);
+      if (nrep.iterations == 0) {
+        if ((st = jacobian_setup(c, o->fp64_blocks)) != 0) goto done;
+        if ((st = assemble_homotopy(c, lambda)) != 0) goto done;
+        if ((st = pdes_pc_setup(c, o->pc_kind)) != 0) goto done;
+      }
+      int kits = 0;
+      double kres = 0;
+      if ((st = gmres_device(c, d_rhs, d_t, o->krylov_reltol, o->krylov_abstol, o->gmres_restart, o->krylov_itermax,
+                             o->jv_mode, &kits, &kres, true)) != 0)
+        goto done;
+      rep->total_krylov_iters += kits;
+      PDES_HIP(hipMemcpyAsync(t.data(), d_t, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      PDES_HIP(hipMemcpyAsync(q.data(), c->d_q, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      PDES_HIP(hipStreamSynchronize(c->stream));
+      // takePredictorStep
+      double tan_norm = std::sqrt(mdot(t, t) + 1.0);
+      for (int64_t i = 0; i < n; ++i) t[i] /= tan_norm;
+      for (int64_t i = 0; i < n; ++i) q0[i] = q[i] - q0[i];
+      const double delta = std::sqrt(mdot(q0, q0));
+      double psi = psi_max;
+      if (iter > 1) {
+        const double arg = std::min(1.0, std::max(-1.0, mdot(t1, t) + 1.0 / (tan_norm * tan_norm_1)));
+        psi = std::acos(arg);
+      }
+      const double fac = std::max(psi / psi_max, std::sqrt(delta / delta_max));
+      h /= fac;
+      h = std::min(h, h_max);
+      lambda = std::max(0.0, lambda - h);
+      if (lambda < eps) lambda = 0.0;
+      for (int64_t i = 0; i < n; ++i) q[i] += h * t[i];
+      PDES_HIP(hipMemcpyAsync(c->d_q, q.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      t1.swap(t);
+      tan_norm_1 = tan_norm;
+    }
+    if ((st = phys_norm(&res_norm)) != 0) goto done;
+    rep->iterations = iter;
+    ++iter;
+  }
+  rep->res_norm = res_norm;
+  rep->converged = (res_norm <= o->res_abstol || res_norm <= rep->res_norm0 * o->res_reltol) ? 1 : 0;
+done:
+  c->hom_lambda = -1.0;
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  hipFree(d_t);
+  hipFree(d_rhs);
   rep->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }

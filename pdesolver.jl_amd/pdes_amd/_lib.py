@@ -56,6 +56,10 @@ _SIGS = {
     "pdes_eval_functional": [_vp, C.c_int, _vp, C.c_int, _dp],
     "pdes_functional_deriv": [_vp, C.c_int, _vp, C.c_int, _vp],
     "pdes_bsr_matvec_transpose": [_vp, _vp, _vp],
+    "pdes_eval_homotopy": [_vp, _vp],
+    "pdes_homotopy_jv": [_vp, _vp, _vp],
+    "pdes_assemble_homotopy_jacobian": [_vp, C.c_double],
+    "pdes_homotopy_solve": [_vp, C.POINTER(abi.NewtonOpts), C.c_double, C.POINTER(abi.NewtonReport)],
     "pdes_adjoint_solve": [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int,
                            C.c_int, _vp, _i32p, _dp],
     "pdes_batch_create": [C.c_int, _vp, _vp, _vp, C.POINTER(_vp)],

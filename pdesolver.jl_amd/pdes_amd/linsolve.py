@@ -112,6 +112,30 @@ def evalResidual_revq(mesh, sbp, eqn: EulerData, opts, input_array, output_array
         dst += out
 
 
+def evalHomotopy(mesh, sbp, eqn: EulerData, opts=None):
+    """evalHomotopy with "FirstOrderDissipation" (src/solver/euler/homotopy.jl:
+    11-31, calcHomotopyDiss :51-249, no boundary term): D(q) at the device
+    state, in the layout of q."""
+    out = np.empty(eqn.q.shape, dtype=np.float64)
+    check(lib().pdes_eval_homotopy(eqn.ctx, out.ctypes.data))
+    return out
+
+
+def evalHomotopy_jv(eqn: EulerData, v):
+    """(dD/dq) v at the device state."""
+    vc = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty(eqn.q.shape, dtype=np.float64)
+    check(lib().pdes_homotopy_jv(eqn.ctx, vc.ctypes.data, out.ctypes.data))
+    return out
+
+
+def calcHomotopyJacobian(mesh, sbp, eqn: EulerData, lam: float, fp64_blocks: bool = False):
+    """Jacobian of (1 - lam) R + lam D into the device BSR matrix
+    (HomotopyMatLO calcLinearOperator, predictor_corrector.jl)."""
+    check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))
+    check(lib().pdes_assemble_homotopy_jacobian(eqn.ctx, float(lam)))
+
+
 FUNCTIONALS = {"lift": 1, "drag": 2, "entropyflux": 3, "massflow": 4, "liftCoefficient": 5, "dragCoefficient": 6}
 
 
@@ -290,4 +314,19 @@ def newton(mesh, sbp, eqn: EulerData, opts: NewtonOptions | None = None):
     o = opts.desc()
     rep = abi.NewtonReport()
     check(lib().pdes_newton_solve(eqn.ctx, C.byref(o), C.byref(rep)))
+    return rep
+
+
+def predictorCorrectorHomotopy(mesh, sbp, eqn: EulerData, opts: NewtonOptions | None = None,
+                               psi_max: float = 10.0 * np.pi / 180.0):
+    """predictorCorrectorHomotopy (src/NonlinearSolvers/predictor_corrector.jl:
+    226-326) with the first-order dissipation as the easy problem, on the
+    device state.  opts gives the final tolerances, itermax (predictor
+    iterations), the preconditioner and the Krylov settings; psi_max is
+    opts["homotopy_psi_max"].  Returns the report (iterations = predictor
+    iterations)."""
+    opts = opts or NewtonOptions()
+    o = opts.desc()
+    rep = abi.NewtonReport()
+    check(lib().pdes_homotopy_solve(eqn.ctx, C.byref(o), float(psi_max), C.byref(rep)))
     return rep
