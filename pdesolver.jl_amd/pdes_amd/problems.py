@@ -107,22 +107,39 @@ def make_problem(kind: str, n: int, p: int, op_kind: str = "omega", flux: str = 
         bc = [PDES_BC_FREESTREAM] * m.numBC
     else:
         raise ValueError(kind)
+    return m, op, _physics(op, flux, bc, Ma, aoa_deg, vortex_Min)
+
+
+def _physics(op, flux, bc, Ma, aoa_deg, vortex_Min):
     if flux == "roe":
-        phys = Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=1,
+        return Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=1,
                        face_flux=PDES_FLUX_ROE, bc_type=bc)
-    elif flux == "es":
+    if flux == "es":
         if op.kind != "omega":
             raise ValueError("the entropy-stable element face integral is for dense-face (Omega) operators")
-        phys = Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=2,
+        return Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=2,
                        face_flux=PDES_FLUX_ESLF, bc_type=bc)
-    elif flux == "ir":
+    if flux == "ir":
         if op.kind != "diage":
             raise ValueError("the IR split form with IRSLF face flux needs a diagonal-E operator")
-        phys = Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=2,
+        return Physics(Ma=Ma, aoa_deg=aoa_deg, vortex_Min=vortex_Min, volume_integral_type=2,
                        face_flux=PDES_FLUX_IRSLF, bc_type=bc)
-    else:
-        raise ValueError(flux)
-    return m, op, phys
+    raise ValueError(flux)
+
+
+def smb_problem(path: str, p: int, op_kind: str = "omega", flux: str = "roe", bc: str = "vortex",
+                bc_tags=None, Ma: float = 0.5, aoa_deg: float = 0.0, vortex_Min: float = 0.95):
+    """A problem on a serial PUMI mesh file (the reference's `smb_name`): the
+    dimension comes from the file, bc "vortex" (isentropicVortexBC, 2D) or
+    "freestream" on every boundary group.  Returns (mesh, op, phys)."""
+    from . import pumi
+    dim = pumi.read_smb(path).dim
+    op = sbpmod.get_operator(dim, p, op_kind)
+    m = pumi.load_smb(path, op, bc_tags=bc_tags)
+    code = {"vortex": PDES_BC_ISENTROPIC_VORTEX, "freestream": PDES_BC_FREESTREAM}[bc]
+    if code == PDES_BC_ISENTROPIC_VORTEX and dim != 2:
+        raise ValueError("the isentropic vortex is two-dimensional")
+    return m, op, _physics(op, flux, [code] * m.numBC, Ma, aoa_deg, vortex_Min)
 
 
 def smb_vortex_problem(path: str, p: int, op_kind: str = "omega", flux: str = "roe",
@@ -131,15 +148,7 @@ def smb_vortex_problem(path: str, p: int, op_kind: str = "omega", flux: str = "r
     on every boundary group (the convergence cases of
     test/euler/convergence/p*/conservative_dg, BC1 => [0, 1, 2, 3]).
     Returns (mesh, op, phys)."""
-    from . import pumi
-    op = sbpmod.get_operator(2, p, op_kind)
-    m = pumi.load_smb(path, op, bc_tags=bc_tags)
-    bc = [PDES_BC_ISENTROPIC_VORTEX] * m.numBC
-    if flux != "roe":
-        raise ValueError("the convergence cases use the Roe face flux")
-    phys = Physics(Ma=0.5, aoa_deg=0.0, vortex_Min=vortex_Min, volume_integral_type=1,
-                   face_flux=PDES_FLUX_ROE, bc_type=bc)
-    return m, op, phys
+    return smb_problem(path, p, op_kind, flux, "vortex", bc_tags, vortex_Min=vortex_Min)
 
 
 def config3_instances(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, amp: float = 1e-3):
