@@ -247,10 +247,15 @@ int pdes_newton_solve(pdes_ctx* ctx, const pdes_newton_opts* opts,
                       pdes_newton_report* report);
 
 /* evalHomotopy with homotopy_function "FirstOrderDissipation"
- * (src/solver/euler/homotopy.jl:11-31 -> calcHomotopyDiss :51-249, without
- * the boundary term, homotopy_addBoundaryIntegrals = false): res[numDof] =
- * D(q) at the device state.  Single (unbatched) contexts, Np <= 64. */
+ * (src/solver/euler/homotopy.jl:11-31 -> calcHomotopyDiss :51-249):
+ * res[numDof] = D(q) at the device state.  Single (unbatched) contexts,
+ * Np <= 64. */
 int pdes_eval_homotopy(pdes_ctx* ctx, double* res);
+
+/* opts "homotopy_addBoundaryIntegrals" (src/input/read_input.jl:438, default
+ * 0) for this context's homotopy dissipation, its products, Jacobian and
+ * solve: the boundary faces add the dissipation against the free stream. */
+int pdes_set_homotopy_boundary(pdes_ctx* ctx, int add_boundary_integrals);
 
 /* b = (dD/dq) v at the device state (the homotopy term of the complex-step
  * product of HomotopyPetscMatFreeLO, predictor_corrector.jl). */

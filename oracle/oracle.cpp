@@ -18,7 +18,7 @@
 //   checkDensity / checkPressure  src/solver/euler/euler.jl:543-611
 //   calcJacobianSparse (complex step, distance-2 coloring)  src/jacobian/jacobian.jl:365-446
 //   SparseMatrixCSC(mesh, Tv, INVISCID, face_type)           src/jacobian/sparse.jl:250-495
-//   calcHomotopyDiss (homotopy_addBoundaryIntegrals = false)  src/solver/euler/homotopy.jl:51-249
+//   calcHomotopyDiss        src/solver/euler/homotopy.jl:51-249
 // The SBP application routines (weakdifferentiate!, interiorfaceinterpolate!,
 // interiorfaceintegrate!, boundaryinterpolate!, boundaryintegrate!) live in
 // the external SummationByParts.jl; they are restated from their documented
@@ -277,18 +277,21 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
 
 // element adjacency through interfaces
 // calcHomotopyDiss (src/solver/euler/homotopy.jl:51-249), the first-order
-// dissipation of the homotopy globalisation, without its boundary term (the
-// default homotopy_addBoundaryIntegrals = false, src/input/read_input.jl:438):
+// dissipation of the homotopy globalisation (bndry: the boundary term of
+// homotopy_addBoundaryIntegrals, default false, src/input/read_input.jl:438):
 //   volume, per reference direction d: t1 = H^-1 Q_d q (differentiateElement!),
 //     t2_j = lambda_max(q_j, dxidx[d, :, j]) t1_j, res += Q_d^T t2
 //     (weakDifferentiateElement!, trans = true);
 //   interfaces: flux_k = lambda_max(avg(qL_k, qR_k), nrm_k) (qL_k - qR_k) / 2 at
 //     each face node (getLambdaMaxSimple, IR_stab.jl:310-325), resL += R_L^T B
 //     flux, resR -= R_R^T B flux (interiorFaceIntegrate!, Add);
+//   boundary faces (bndry only): the same flux between the face state and the
+//     free stream (calcFreeStream, whatever the BC), res += R^T B flux
+//     (boundaryFaceIntegrate!, Add);
 //   then res = -res.
 template <class T>
 void homotopy_diss(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph, const T* q,
-                   T* res) {
+                   T* res, bool bndry) {
   const int dim = m->dim, nd = dim + 2, Np = s->Np, Nf = s->Nf, nfaces = dim + 1;
   Params p = make_params(ph, dim);
   const size_t n = (size_t)m->numEl * Np * nd;
@@ -351,6 +354,32 @@ void homotopy_diss(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_p
       }
     }
   }
+  if (bndry) {
+    double qg_d[5];
+    calcFreeStream(p, qg_d);
+    for (int bi = 0; bi < m->numBoundaryFaces; ++bi) {
+      const int e = m->bndryfaces[2 * bi], f = m->bndryfaces[2 * bi + 1];
+      for (int kf = 0; kf < Nf; ++kf) {
+        T qa[5];
+        for (int k = 0; k < nd; ++k) {
+          T vl = 0.0;
+          for (int st = 0; st < s->stencil; ++st)
+            vl += s->interp[st * Nf + kf] * q[((size_t)e * Np + s->perm[st * nfaces + f]) * nd + k];
+          qL[kf * nd + k] = vl;
+          qa[k] = 0.5 * (vl + qg_d[k]);
+        }
+        const T lam = getLambdaMax(p, qa, m->nrm_bndry + ((size_t)bi * Nf + kf) * dim);
+        for (int k = 0; k < nd; ++k) ff[kf * nd + k] = 0.5 * lam * (qL[kf * nd + k] - qg_d[k]);
+      }
+      for (int st = 0; st < s->stencil; ++st) {
+        const int node = s->perm[st * nfaces + f];
+        for (int kf = 0; kf < Nf; ++kf) {
+          const double fac = s->interp[st * Nf + kf] * s->wface[kf];
+          for (int k = 0; k < nd; ++k) res[((size_t)e * Np + node) * nd + k] += fac * ff[kf * nd + k];
+        }
+      }
+    }
+  }
   for (size_t i = 0; i < n; ++i) res[i] = -res[i];
 }
 
@@ -400,18 +429,18 @@ int oracle_jv_complex_step(const pdes_mesh_desc* m, const pdes_sbp_desc* s, cons
 }
 
 int oracle_homotopy(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph, const double* q,
-                    double* res) {
-  homotopy_diss<double>(m, s, ph, q, res);
+                    int bndry, double* res) {
+  homotopy_diss<double>(m, s, ph, q, res, bndry != 0);
   return PDES_OK;
 }
 
 // directional derivative of calcHomotopyDiss by the complex step
 int oracle_homotopy_jv(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
-                       const double* q, const double* v, double h, double* b) {
+                       const double* q, const double* v, double h, int bndry, double* b) {
   size_t n = (size_t)m->numEl * s->Np * (m->dim + 2);
   std::vector<cplx> qc(n), r(n);
   for (size_t i = 0; i < n; ++i) qc[i] = cplx(q[i], h * v[i]);
-  homotopy_diss<cplx>(m, s, ph, qc.data(), r.data());
+  homotopy_diss<cplx>(m, s, ph, qc.data(), r.data(), bndry != 0);
   for (size_t i = 0; i < n; ++i) b[i] = r[i].imag() / h;
   return PDES_OK;
 }

@@ -78,20 +78,22 @@ def jv_complex_step(b, q, v, h=1e-20):
     return out
 
 
-def homotopy(b, q):
-    """calcHomotopyDiss (src/solver/euler/homotopy.jl:51-249), no boundary term."""
+def homotopy(b, q, bndry=False):
+    """calcHomotopyDiss (src/solver/euler/homotopy.jl:51-249); bndry adds the
+    boundary term of homotopy_addBoundaryIntegrals."""
     q = np.ascontiguousarray(q, dtype=np.float64)
     out = np.zeros_like(q)
-    _check(lib().oracle_homotopy(C.byref(b.mesh), C.byref(b.sbp), C.byref(b.phys), _d(q), _d(out)), "homotopy")
+    _check(lib().oracle_homotopy(C.byref(b.mesh), C.byref(b.sbp), C.byref(b.phys), _d(q), C.c_int(int(bndry)),
+                                 _d(out)), "homotopy")
     return out
 
 
-def homotopy_jv(b, q, v, h=1e-20):
+def homotopy_jv(b, q, v, h=1e-20, bndry=False):
     q = np.ascontiguousarray(q, dtype=np.float64)
     v = np.ascontiguousarray(v, dtype=np.float64)
     out = np.zeros_like(q)
     _check(lib().oracle_homotopy_jv(C.byref(b.mesh), C.byref(b.sbp), C.byref(b.phys), _d(q), _d(v),
-                                    C.c_double(h), _d(out)), "homotopy jv")
+                                    C.c_double(h), C.c_int(int(bndry)), _d(out)), "homotopy jv")
     return out
 
 

@@ -58,6 +58,7 @@ struct pdes_ctx {
   // homotopy (predictorCorrectorHomotopy): when hom_lambda >= 0 the Newton
   // system is H = (1 - lambda) R + lambda D with D the first-order dissipation
   double hom_lambda = -1.0;
+  int hom_bndry = 0;          // homotopy_addBoundaryIntegrals
   double* d_w = nullptr;      // SBP weights H (uploaded on first homotopy use)
   // packed batches: per-instance 0/1 mask honoured by the BSR product and the
   // preconditioner apply (instances with 0 are skipped); null = all

@@ -6,8 +6,10 @@
 // lambda_max(q_j, dxidx[d, :, j]) t1_j (getLambdaMax, euler_funcs.jl:1887-1910),
 // D -= Q_d^T t2; per interior face node, flux = lambda_max(avg, nrm)
 // (qL - qR) / 2 (getLambdaMaxSimple, IR_stab.jl:310-325), D_L -= R_L^T B flux,
-// D_R += R_R^T B flux.  The boundary term (homotopy_addBoundaryIntegrals) is
-// off, its default (src/input/read_input.jl:438).
+// D_R += R_R^T B flux.  With homotopy_addBoundaryIntegrals (default off,
+// src/input/read_input.jl:438) each boundary face node adds the same flux
+// between its state and the free stream (calcFreeStream, whatever the BC):
+// D -= R^T B flux.
 //
 // One block of 64 threads per element, thread i owns volume node i; the
 // element's states, the scaled derivatives t2 and the face contributions are
@@ -50,8 +52,8 @@ template <> struct HomOut<dual<1>> {
 template <int DIM, class T>
 __global__ void __launch_bounds__(64)
 homotopy_diss_kernel(int np, int nf, int st, DevSBP sbp, DevMesh mesh, const double* __restrict__ w,
-                     PhysConst phys, const double* __restrict__ q, const double* __restrict__ v, double a, double b,
-                     double* __restrict__ out, double* __restrict__ jv) {
+                     PhysConst phys, int bndry, const double* __restrict__ q, const double* __restrict__ v,
+                     double a, double b, double* __restrict__ out, double* __restrict__ jv) {
   constexpr int ND = DIM + 2, NFACES = DIM + 1;
   extern __shared__ __align__(16) unsigned char hom_smem[];
   T* qs = reinterpret_cast<T*>(hom_smem);
@@ -101,8 +103,28 @@ homotopy_diss_kernel(int np, int nf, int st, DevSBP sbp, DevMesh mesh, const dou
     const int k = slot - f * nf;
     const ElFace rec = mesh.elface[e * NFACES + f];
     if (rec.x < 0) {
+      if (!bndry) {
 #pragma unroll
-      for (int c = 0; c < ND; ++c) G[(f * nf + k) * ND + c] = 0.0;
+        for (int c = 0; c < ND; ++c) G[(f * nf + k) * ND + c] = 0.0;
+        continue;
+      }
+      T qb[ND], qa[ND];
+      double qg[ND];
+      free_stream<DIM>(phys, qg);
+#pragma unroll
+      for (int c = 0; c < ND; ++c) qb[c] = 0.0;
+      for (int s = 0; s < st; ++s) {
+        const double r = sbp.interp[s * nf + k];
+        const int node = sbp.perm[s * NFACES + f];
+#pragma unroll
+        for (int c = 0; c < ND; ++c) qb[c] = qb[c] + r * qs[node * ND + c];
+      }
+#pragma unroll
+      for (int c = 0; c < ND; ++c) qa[c] = 0.5 * (qb[c] + qg[c]);
+      const T lam = hom_lambda_max<DIM>(phys, qa, mesh.nrm_bndry + ((int64_t)rec.y * nf + k) * DIM);
+      const double wgt = -sbp.wface[k];
+#pragma unroll
+      for (int c = 0; c < ND; ++c) G[(f * nf + k) * ND + c] = wgt * (0.5 * lam * (qb[c] - qg[c]));
       continue;
     }
     const int64_t nb = rec.x;
@@ -159,7 +181,7 @@ static int launch_hom(pdes_ctx* c, const double* q, const double* v, double a, d
   constexpr int ND = DIM + 2;
   const size_t lds = (size_t)(2 * c->Np * ND + (DIM + 1) * c->Nf * ND) * sizeof(T);
   hipLaunchKernelGGL((homotopy_diss_kernel<DIM, T>), dim3((unsigned)c->numEl), dim3(64), lds, c->stream, c->Np,
-                     c->Nf, c->stencil, c->devsbp(), c->devmesh(), c->d_w, c->pc, q, v, a, b, out, jv);
+                     c->Nf, c->stencil, c->devsbp(), c->devmesh(), c->d_w, c->pc, c->hom_bndry, q, v, a, b, out, jv);
   PDES_HIP(hipGetLastError());
   return PDES_OK;
 }
@@ -194,6 +216,12 @@ int pdes_eval_homotopy(pdes_ctx* c, double* res) {
   if (st) return st;
   PDES_HIP(hipMemcpyAsync(res, c->d_work, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_set_homotopy_boundary(pdes_ctx* c, int add_boundary_integrals) {
+  PDES_CHECK(c, PDES_ERR_ARG, "null context");
+  c->hom_bndry = add_boundary_integrals ? 1 : 0;
   return PDES_OK;
 }
 

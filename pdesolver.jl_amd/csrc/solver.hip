@@ -1148,12 +1148,25 @@ int pdes_homotopy_solve(pdes_ctx* c, const pdes_newton_opts* o, double psi_max, 
   *rep = pdes_newton_report{};
   const int64_t n = c->numDof;
   const double eps = 2.220446049250313e-16;
-  std::vector<double> M(n), t(n), t1(n), q(n), q0(n), rhs(n);
-  double* d_t = nullptr;
-  double* d_rhs = nullptr;
+  std::vector<double> M(n), t(n), t1(n), q(n), q0(n);
+  // every exit, the early returns of the HIP checks included, leaves the
+  // context solving the physics again and frees the workspace
+  struct Guard {
+    pdes_ctx* c;
+    double* d_t = nullptr;
+    double* d_rhs = nullptr;
+    ~Guard() {
+      c->hom_lambda = -1.0;
+      hipStreamSynchronize(c->stream);
+      if (d_t) hipFree(d_t);
+      if (d_rhs) hipFree(d_rhs);
+    }
+  } guard{c};
   int st = PDES_OK;
-  PDES_CHECK(hipMalloc((void**)&d_t, n * sizeof(double)) == hipSuccess &&
-             hipMalloc((void**)&d_rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "homotopy workspace");
+  PDES_CHECK(hipMalloc((void**)&guard.d_t, n * sizeof(double)) == hipSuccess &&
+             hipMalloc((void**)&guard.d_rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "homotopy workspace");
+  double* const d_t = guard.d_t;
+  double* const d_rhs = guard.d_rhs;
   auto phys_norm = [&](double* out) -> int {
     c->hom_lambda = -1.0;
     int s2 = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, c->phys.check_positivity != 0);
@@ -1242,10 +1255,6 @@ int pdes_homotopy_solve(pdes_ctx* c, const pdes_newton_opts* o, double psi_max, 
   rep->res_norm = res_norm;
   rep->converged = (res_norm <= o->res_abstol || res_norm <= rep->res_norm0 * o->res_reltol) ? 1 : 0;
 done:
-  c->hom_lambda = -1.0;
-  PDES_HIP(hipStreamSynchronize(c->stream));
-  hipFree(d_t);
-  hipFree(d_rhs);
   rep->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }

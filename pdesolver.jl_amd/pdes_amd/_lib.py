@@ -57,6 +57,7 @@ _SIGS = {
     "pdes_functional_deriv": [_vp, C.c_int, _vp, C.c_int, _vp],
     "pdes_bsr_matvec_transpose": [_vp, _vp, _vp],
     "pdes_eval_homotopy": [_vp, _vp],
+    "pdes_set_homotopy_boundary": [_vp, C.c_int],
     "pdes_homotopy_jv": [_vp, _vp, _vp],
     "pdes_assemble_homotopy_jacobian": [_vp, C.c_double],
     "pdes_homotopy_solve": [_vp, C.POINTER(abi.NewtonOpts), C.c_double, C.POINTER(abi.NewtonReport)],

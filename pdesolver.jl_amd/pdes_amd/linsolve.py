@@ -112,10 +112,16 @@ def evalResidual_revq(mesh, sbp, eqn: EulerData, opts, input_array, output_array
         dst += out
 
 
+def setHomotopyBoundaryIntegrals(eqn: EulerData, on: bool):
+    """opts["homotopy_addBoundaryIntegrals"] for this context's homotopy
+    dissipation (default off)."""
+    check(lib().pdes_set_homotopy_boundary(eqn.ctx, int(bool(on))))
+
+
 def evalHomotopy(mesh, sbp, eqn: EulerData, opts=None):
     """evalHomotopy with "FirstOrderDissipation" (src/solver/euler/homotopy.jl:
-    11-31, calcHomotopyDiss :51-249, no boundary term): D(q) at the device
-    state, in the layout of q."""
+    11-31, calcHomotopyDiss :51-249): D(q) at the device state, in the layout
+    of q (boundary term per setHomotopyBoundaryIntegrals)."""
     out = np.empty(eqn.q.shape, dtype=np.float64)
     check(lib().pdes_eval_homotopy(eqn.ctx, out.ctypes.data))
     return out
@@ -318,15 +324,16 @@ def newton(mesh, sbp, eqn: EulerData, opts: NewtonOptions | None = None):
 
 
 def predictorCorrectorHomotopy(mesh, sbp, eqn: EulerData, opts: NewtonOptions | None = None,
-                               psi_max: float = 10.0 * np.pi / 180.0):
+                               psi_max: float = 10.0 * np.pi / 180.0, addBoundaryIntegrals: bool = False):
     """predictorCorrectorHomotopy (src/NonlinearSolvers/predictor_corrector.jl:
     226-326) with the first-order dissipation as the easy problem, on the
     device state.  opts gives the final tolerances, itermax (predictor
     iterations), the preconditioner and the Krylov settings; psi_max is
     opts["homotopy_psi_max"].  Returns the report (iterations = predictor
-    iterations)."""
+    iterations).  addBoundaryIntegrals is opts["homotopy_addBoundaryIntegrals"]."""
     opts = opts or NewtonOptions()
     o = opts.desc()
     rep = abi.NewtonReport()
+    setHomotopyBoundaryIntegrals(eqn, addBoundaryIntegrals)
     check(lib().pdes_homotopy_solve(eqn.ctx, C.byref(o), float(psi_max), C.byref(rep)))
     return rep

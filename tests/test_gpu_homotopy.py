@@ -36,8 +36,18 @@ def test_dissipation_matches_oracle(oracle, kind, n, p, opk, flux):
     jv = linsolve.evalHomotopy_jv(eqn, v)
     jref = oracle.homotopy_jv(b, q, v)
     assert np.abs(jv - jref).max() <= 1e-11 * np.abs(jref).max()
+    # homotopy_addBoundaryIntegrals
+    linsolve.setHomotopyBoundaryIntegrals(eqn, True)
+    d = linsolve.evalHomotopy(m, op, eqn)
+    ref = oracle.homotopy(b, q, bndry=True)
+    assert np.abs(d - ref).max() <= 1e-12 * np.abs(ref).max()
+    jv = linsolve.evalHomotopy_jv(eqn, v)
+    jref = oracle.homotopy_jv(b, q, v, bndry=True)
+    assert np.abs(jv - jref).max() <= 1e-11 * np.abs(jref).max()
     # uniform flow: zero (test_homotopy.jl)
     eqn.set_q(problems.freestream((m.numEl, op.numnodes), m.dim, ph))
+    assert np.abs(linsolve.evalHomotopy(m, op, eqn)).max() <= 1e-13
+    linsolve.setHomotopyBoundaryIntegrals(eqn, False)
     assert np.abs(linsolve.evalHomotopy(m, op, eqn)).max() <= 1e-13
     eqn.free()
 
@@ -57,8 +67,22 @@ def test_homotopy_jacobian_matches_products(oracle, kind, n, p, opk, flux, lam):
     eqn.free()
 
 
-@pytest.mark.parametrize("pc", [abi.PDES_PC_BJACOBI, abi.PDES_PC_ILU0])
-def test_predictor_corrector_reaches_newton_solution(oracle, pc):
+def test_homotopy_jacobian_with_boundary_term(oracle):
+    m, op, ph, b, q, eqn = _eqn("vortex", 3, 2, "omega", "roe", seed=2)
+    linsolve.setHomotopyBoundaryIntegrals(eqn, True)
+    lam = 0.35
+    linsolve.calcHomotopyJacobian(m, op, eqn, lam, fp64_blocks=True)
+    v = np.random.default_rng(8).standard_normal(q.shape)
+    out = np.zeros_like(v)
+    linsolve.HIPBSRLO(eqn, fp64_blocks=True).applyLinearOperator(m, op, eqn, None, None, 0.0, v, out)
+    ref = (1 - lam) * oracle.jv_complex_step(b, q, v) + lam * oracle.homotopy_jv(b, q, v, bndry=True)
+    assert np.abs(out - ref).max() <= 1e-11 * np.abs(ref).max()
+    eqn.free()
+
+
+@pytest.mark.parametrize("pc,bndry", [(abi.PDES_PC_BJACOBI, False), (abi.PDES_PC_ILU0, False),
+                                     (abi.PDES_PC_ILU0, True)])
+def test_predictor_corrector_reaches_newton_solution(oracle, pc, bndry):
     """Vortex, Omega p=2, Roe, from the exact solution with 5 % noise: the
     predictor-corrector path from lambda = 1 to 0 converges to ||R|| <= 1e-10
     and to the oracle's direct-solve Newton state."""
@@ -70,7 +94,7 @@ def test_predictor_corrector_reaches_newton_solution(oracle, pc):
     eqn.set_q(q0)
     opts = linsolve.NewtonOptions(res_abstol=1e-10, res_reltol=1e-300, itermax=60, krylov_reltol=1e-6,
                                   fp64_blocks=True, pc_kind=pc, gmres_restart=60, krylov_itermax=3000)
-    rep = linsolve.predictorCorrectorHomotopy(m, op, eqn, opts)
+    rep = linsolve.predictorCorrectorHomotopy(m, op, eqn, opts, addBoundaryIntegrals=bndry)
     print("predictor iterations", rep.iterations, "residual", rep.res_norm, "krylov", rep.total_krylov_iters,
           flush=True)
     assert rep.converged and rep.res_norm <= 1e-10

@@ -88,11 +88,34 @@ def test_piecewise_constant_face_term(oracle, kind, n, p, opk):
     assert np.abs(d.sum(axis=1) - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("kind,n,p,opk", CASES)
+def test_boundary_term(oracle, kind, n, p, opk):
+    """homotopy_addBoundaryIntegrals: zero on the free stream itself; on an
+    element-wise constant field each element's sum gains minus the jump to
+    the free stream over its boundary faces, weighted by wface lambda / 2."""
+    m, op, ph, b = _make(kind, n, p, opk)
+    qinf = problems.freestream((1,), m.dim, ph)[0]
+    q0 = np.broadcast_to(qinf, (m.numEl, op.numnodes, m.dim + 2)).copy()
+    assert np.abs(oracle.homotopy(b, q0, bndry=True)).max() <= 1e-13
+    rng = np.random.default_rng(6)
+    qe = qinf[None, :] * (1 + 0.05 * rng.uniform(-1, 1, (m.numEl, m.dim + 2)))
+    q = np.repeat(qe[:, None, :], op.numnodes, axis=1)
+    d0 = oracle.homotopy(b, q)
+    d1 = oracle.homotopy(b, q, bndry=True)
+    ref = np.zeros((m.numEl, m.dim + 2))
+    wf = op.face.wface
+    for bi, (e, f) in enumerate(m.bndryfaces):
+        lam = _lam(ph, np.broadcast_to(0.5 * (qe[e] + qinf), (len(wf), m.dim + 2)), m.nrm_bndry[bi])
+        ref[e] -= np.outer(wf * lam, qe[e] - qinf).sum(0) / 2
+    assert np.abs((d1 - d0).sum(axis=1) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
 def test_homotopy_jv_matches_finite_difference(oracle):
     m, op, ph, b = _make("vortex", 3, 2, "omega")
     q = state_for("vortex", m, ph)
     v = np.random.default_rng(5).standard_normal(q.shape)
-    jv = oracle.homotopy_jv(b, q, v)
     h = 1e-6
-    fd = (oracle.homotopy(b, q + h * v) - oracle.homotopy(b, q - h * v)) / (2 * h)
-    assert np.abs(jv - fd).max() <= 1e-6 * np.abs(jv).max()
+    for bndry in (False, True):
+        jv = oracle.homotopy_jv(b, q, v, bndry=bndry)
+        fd = (oracle.homotopy(b, q + h * v, bndry=bndry) - oracle.homotopy(b, q - h * v, bndry=bndry)) / (2 * h)
+        assert np.abs(jv - fd).max() <= 1e-6 * np.abs(jv).max()
