@@ -83,7 +83,8 @@ def read_smb(path: str) -> SmbMesh:
     for t in range(1, _NTYPES):
         if counts[t]:
             down[t] = w.take(int(counts[t]) * _DOWN[t], ">u4").astype(np.int64).reshape(-1, _DOWN[t])
-            if down[t].size and down[t].max() >= counts[_TYPE_DIM[t] - 1 if t != _TET else _TRI]:
+            lower = {_EDGE: 0, _TRI: _EDGE, _TET: _TRI}.get(t)
+            if lower is not None and down[t].size and down[t].max() >= counts[lower]:
                 raise ValueError(f"{path}: downward id out of range for type {t}")
     nv = int(counts[0])
     coords = w.take(3 * nv, ">f8").reshape(nv, 3).copy()
