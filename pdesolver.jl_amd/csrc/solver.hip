@@ -1057,10 +1057,23 @@ int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report
   const int64_t n = c->numDof;
   int st;
   double* res = c->d_res;
-  double* dq = nullptr;
-  double* rhs = nullptr;
-  PDES_CHECK(hipMalloc((void**)&dq, n * sizeof(double)) == hipSuccess &&
-             hipMalloc((void**)&rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "Newton workspace");
+  // every exit, the early returns of the HIP checks included, turns the
+  // implicit-Euler term off and frees the workspace
+  struct Guard {
+    pdes_ctx* c;
+    double* dq = nullptr;
+    double* rhs = nullptr;
+    ~Guard() {
+      c->ptc_on = false;
+      hipStreamSynchronize(c->stream);
+      if (dq) hipFree(dq);
+      if (rhs) hipFree(rhs);
+    }
+  } guard{c};
+  PDES_CHECK(hipMalloc((void**)&guard.dq, n * sizeof(double)) == hipSuccess &&
+             hipMalloc((void**)&guard.rhs, n * sizeof(double)) == hipSuccess, PDES_ERR_NOMEM, "Newton workspace");
+  double* const dq = guard.dq;
+  double* const rhs = guard.rhs;
   auto norm_res = [&](double* out) -> int {
     int s2 = launch_residual(c, c->d_q, nullptr, res, nullptr, c->phys.check_positivity != 0);
     if (!s2 && c->hom_lambda >= 0.0)
@@ -1120,9 +1133,6 @@ int pdes_newton_solve(pdes_ctx* c, const pdes_newton_opts* o, pdes_newton_report
     }
   }
 fin:
-  c->ptc_on = false;
-  hipFree(dq);
-  hipFree(rhs);
   rep->elapsed_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }

@@ -187,6 +187,37 @@ def test_irslf_consistency_and_dissipation(oracle):
     assert np.dot(wL - wR, diss) > 0
 
 
+def test_ir_flux_entropy_conservation(oracle):
+    """Tadmor's condition, which fixes the Ismail-Roe flux's averages: with
+    the entropy variables w of U = -rho s / (gamma - 1) and the potential
+    psi = rho u . n, (w_R - w_L) . F_IR(qL, qR; n) = psi_R - psi_L for any
+    pair of states, in 2D and 3D, on both logavg branches; the IRSLF flux
+    makes the same product smaller (entropy stable)."""
+    rng = np.random.default_rng(11)
+    for dim in (2, 3):
+        nd = dim + 2
+        for scale in (1e-4, 1e-1):
+            for _ in range(20):
+                rho, p = rng.uniform(0.5, 2.0, 2)
+                u = rng.uniform(-0.5, 0.5, dim)
+                qL = np.concatenate([[rho], rho * u, [p / 0.4 + 0.5 * rho * (u @ u)]])
+                qR = qL * (1 + scale * rng.uniform(-1, 1, nd))
+                n = rng.standard_normal(dim)
+                wL, wR = np.zeros(nd), np.zeros(nd)
+                oracle.lib().oracle_ir_vars(dim, C.byref(PH), _p(qL), _p(wL))
+                oracle.lib().oracle_ir_vars(dim, C.byref(PH), _p(qR), _p(wR))
+                F = ir(oracle, qL, qR, n[:, None])[0]
+                psi = lambda q: q[1:dim + 1] @ n
+                lhs = (wR - wL) @ F
+                rhs = psi(qR) - psi(qL)
+                # w_R - w_L cancels: the rounding scale is |w| |F|
+                tol = 1e-14 * ((np.abs(wL) + np.abs(wR)) @ np.abs(F) + np.abs(qL[1:dim + 1] @ n))
+                assert abs(lhs - rhs) <= tol, (dim, scale, lhs, rhs)
+                if dim == 2:
+                    Fs = irslf(oracle, qL, qR, n)
+                    assert (wR - wL) @ Fs <= rhs + tol
+
+
 def test_logavg_branches_and_complex_step(oracle):
     # logavg (bc_solvers.jl:942-956): both branches agree with the exact
     # log mean; derivative by complex step vs analytic (test_jac.jl:2001-2025)
