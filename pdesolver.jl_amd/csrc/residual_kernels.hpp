@@ -280,11 +280,12 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   } else {
     // split form (calcVolumeIntegralsSplitFormLinear): per-node IR z-variables in LDS,
     // each unordered node pair evaluated once by the node that starts its circulant distance.
+    // this node's z stays in registers for the pair rounds
+    T zi[ND];
     if (active) {
-      T z[ND];
-      ir_zvars<DIM>(phys, qi, z);
+      ir_zvars<DIM>(phys, qi, zi);
 #pragma unroll
-      for (int k = 0; k < ND; ++k) zpl[k * PL + me + i] = z[k];
+      for (int k = 0; k < ND; ++k) zpl[k * PL + me + i] = zi[k];
     }
     __syncthreads();
     // circulant distance dd: node i evaluates the pair (i, i+dd mod NP); the
@@ -304,12 +305,16 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         if (!(half && i >= NP / 2)) {
           int kk = i + dd;
           if (kk >= NP) kk -= NP;
-          // reference orientation: F(q_j, q_k) with j > k, direction rows of
-          // dxidx at node j.  The states are read from LDS in that order
-          // (lanes of one wave hold both orientations; no branch, no select).
+          // the reference evaluates F(q_j, q_k) with j > k and the direction
+          // rows of dxidx at node j.  The IR flux is symmetric in its two
+          // states (on the logavg series branch to the last bit: the sums
+          // commute and f changes sign only), so the plain kernel uses this
+          // node's own z as the left state and reads only the partner's from
+          // LDS.  The RK-stage kernel, whose epilogue holds more registers,
+          // reads both states from LDS in the reference's order (lanes of
+          // one wave hold both orientations; no branch, no select): keeping
+          // z in registers there spills.  The direction takes node j's metric.
           const bool i_is_j = i > kk;
-          const int jnode = i_is_j ? i : kk;
-          const int knode = i_is_j ? kk : i;
           // sum_d 2 S_d[i][kk] F_d is the IR flux in the one direction
           // n = sum_d 2 S_d[i][kk] dxidx[d, :] of node j (the flux is linear in
           // the direction); node j is this node, or the metric is the
@@ -332,13 +337,22 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
               for (int d = 1; d < DIM; ++d) n[x] = n[x] + s2[d] * mk[x * DIM + d];
             }
           }
-          T zL[ND], zR[ND];
+          if constexpr (RK) {
+            const int jnode = i_is_j ? i : kk;
+            const int knode = i_is_j ? kk : i;
+            T zL[ND], zR[ND];
 #pragma unroll
-          for (int k = 0; k < ND; ++k) {
-            zL[k] = zpl[k * PL + me + jnode];
-            zR[k] = zpl[k * PL + me + knode];
+            for (int k = 0; k < ND; ++k) {
+              zL[k] = zpl[k * PL + me + jnode];
+              zR[k] = zpl[k * PL + me + knode];
+            }
+            ir_flux_z<DIM, 1>(phys, zL, zR, n, g);
+          } else {
+            T zR[ND];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) zR[k] = zpl[k * PL + me + kk];
+            ir_flux_z<DIM, 1>(phys, zi, zR, n, g);
           }
-          ir_flux_z<DIM, 1>(phys, zL, zR, n, g);
           // res_i -= 2 S[i][kk] F, and the partner gets -2 S[kk][i] F = +g
 #pragma unroll
           for (int k = 0; k < ND; ++k) ri[k] = ri[k] - g[k];
