@@ -185,7 +185,8 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r,
 //   rho_hat = z1_avg logavg(z4) = s0 s3 / (4 F3),   p1_hat = s3 r0,
 //   p2_hat = p1_hat (c1 F0/F3 + c2)   (c1 = (gamma+1)/(2 gamma), c2 = (gamma-1)/(2 gamma)),
 //   h_hat = gamma/(gamma-1) p2_hat/rho_hat + |v|^2/2 = 4 gamma/(gamma-1) r0^2 (c1 F0 + c2 F3) + |v|^2/2.
-// The three reciprocals (r0, r3, 1/F3) are drcp's Newton-refined estimates.
+// r0 and r3 come from one drcp of s0 s3 and 1/F3 from another (Newton-refined
+// hardware estimates).
 // lzL / lzR: optional logs {log z1, log z4} of the two states (pointers to
 // log z1, with log z4 LS elements further; read only on the log branch of
 // logavg), or nullptr.  dl: optional output {log(zL1/zR1), log(zL4/zR4)},
@@ -198,8 +199,10 @@ __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
   const T s0 = zL[0] + zR[0];
   const T s3 = zL[ND - 1] + zR[ND - 1];
-  const T r0 = drcp(s0);
-  const T r3 = drcp(s3);
+  // one refined reciprocal for both: 1/s0 = s3 t, 1/s3 = s0 t with t = 1/(s0 s3)
+  const T t03 = drcp(s0 * s3);
+  const T r0 = s3 * t03;
+  const T r3 = s0 * t03;
   T F0, F3;
   if constexpr (std::is_same<LP, std::nullptr_t>::value) {
     F0 = logavg_factor(zL[0], zR[0], r0, nullptr, nullptr);
