@@ -12,12 +12,14 @@ from helpers import setup, state_for
 
 pytestmark = pytest.mark.gpu
 
-CASES = [("vortex", 3, 1, "omega"), ("vortex", 3, 2, "omega"), ("vortex", 3, 2, "diage"), ("cube", 1, 2, "omega")]
+CASES = [("vortex", 3, 1, "omega"), ("vortex", 3, 2, "omega"), ("vortex", 3, 2, "diage"), ("cube", 1, 2, "omega"),
+         ("vortex", 3, 2, "omega-es")]
 
 
 def _problem(kind, n, p, opk):
-    flux = "ir" if opk == "diage" else "roe"
-    m, op, ph = problems.make_problem(kind, n, p, opk, flux, Ma=0.5, aoa_deg=3.0)
+    # "omega-es": split-form volume with the entropy-stable element face integral
+    flux = {"diage": "ir", "omega-es": "es"}.get(opk, "roe")
+    m, op, ph = problems.make_problem(kind, n, p, opk.split("-")[0], flux, Ma=0.5, aoa_deg=3.0)
     q = problems.vortex_noise_state(m, ph, seed=2) if kind == "vortex" else problems.freestream_bump_state(m, ph)
     return m, op, ph, q
 
