@@ -160,7 +160,10 @@ int pdes_copy_bandwidth(int64_t nbytes, int iters, double* avg_ms);
 /* Free and total device memory in bytes (hipMemGetInfo), for footprint checks. */
 int pdes_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 int pdes_rk4(pdes_ctx* ctx, int nsteps, double h, double* elapsed_ms);
+/* b_dev = (dR/dq) v_dev at the device state, device vectors of numDof doubles
+ * (the device-pointer form of pdes_jv_complex_step). */
 int pdes_jv_device(pdes_ctx* ctx, const double* v_dev, double* b_dev);
+/* Waits for the context's stream. */
 int pdes_sync(pdes_ctx* ctx);
 
 /* calcNorm(eqn, res, strongres=true)  src/Utils/Utils.jl:427-449 on the device
@@ -183,9 +186,9 @@ int pdes_get_bsr_pattern(pdes_ctx* ctx, int64_t* rowptr, int32_t* colind);
 int pdes_assemble_jacobian(pdes_ctx* ctx);
 /* calcJacobianSparse (src/jacobian/jacobian.jl:365-446): the same BSR values
  * from directional derivatives of the residual over a distance-2 element
- * colouring (numColours x Np x ndof derivative evaluations).  Used by
- * pdes_assemble_jacobian for the entropy-stable element face integral, and as
- * an independent check of the element-wise Jacobian kernels. */
+ * colouring (numColours x Np x ndof derivative evaluations).  An independent
+ * check of the element-wise Jacobian kernels (pdes_assemble_jacobian), and
+ * the derivative path of the homotopy Jacobian. */
 int pdes_assemble_jacobian_coloring(pdes_ctx* ctx);
 int pdes_get_bsr_values(pdes_ctx* ctx, void* values_host);
 /* b = J x with the assembled BSR matrix (applyLinearOperator of PetscMatLO,
@@ -193,17 +196,20 @@ int pdes_get_bsr_values(pdes_ctx* ctx, void* values_host);
 int pdes_bsr_matvec(pdes_ctx* ctx, const double* x_host, double* b_host);
 
 /* Preconditioner (AbstractPC, src/linearsolvers/LinearSolvers.jl:185-400):
- * kind 0 = element block-Jacobi (NewtonBDiagPC, preconditioning_types.jl:41),
- * kind 1 = ILU(0) on the BSR matrix.  Factor and apply in FP32. */
+ * PDES_PC_NONE, PDES_PC_BJACOBI = element block-Jacobi (NewtonBDiagPC,
+ * preconditioning_types.jl:41), PDES_PC_ILU0 = element-block ILU(0) on the
+ * BSR matrix.  Factor and apply in FP32. */
 #define PDES_PC_NONE 0
 #define PDES_PC_BJACOBI 1
 #define PDES_PC_ILU0 2
 int pdes_pc_setup(pdes_ctx* ctx, int kind);
 int pdes_pc_apply(pdes_ctx* ctx, const double* b_host, double* x_host);
 
-/* GMRES(restart) with right preconditioning and modified Gram-Schmidt
- * (PETSc KSP options of src/input/read_input.jl:560-581) on the device.
- * mode 0: assembled BSR matvec, mode 1: matrix-free complex-step Jv. */
+/* GMRES(restart) with right preconditioning (PETSc KSP options of
+ * src/input/read_input.jl:560-581) on the device; classical Gram-Schmidt
+ * with one reorthogonalisation, convergence decided on the true residual
+ * ||b - A x|| <= max(rtol ||b - A x0||, atol).  mode 0: assembled BSR matvec,
+ * mode 1: matrix-free complex-step Jv. */
 int pdes_gmres(pdes_ctx* ctx, const double* b_host, double* x_host, double rtol,
                double atol, int restart, int maxit, int mode, int* iters,
                double* final_resnorm);
