@@ -217,8 +217,9 @@ class PCNone(AbstractPC):
 
 
 class HIPBJacobiPC(AbstractPC):
-    """Element block-Jacobi: LU of each element's (Np*ndof)^2 diagonal block,
-    FP32 factor and apply."""
+    """Element block-Jacobi (NewtonBDiagPC): the explicit inverse of each
+    element's (Np*ndof)^2 diagonal block by Gauss-Jordan elimination with
+    partial pivoting, in FP32, applied as one product per element."""
     kind = abi.PDES_PC_BJACOBI
 
     def __init__(self, eqn: EulerData):
