@@ -91,6 +91,10 @@ def test_reader_rejects_bad_files(tmp_path):
         pumi.load_smb(_path("squarevortex_small0.smb"), op, bc_tags=[[0, 1]])
     with pytest.raises(ValueError, match="3D mesh"):
         pumi.load_smb(_path("tet8cube0.smb"), op)
+    with pytest.raises(ValueError, match="two boundary groups"):
+        pumi.load_smb(_path("squarevortex_small0.smb"), op, bc_tags=[[0, 1], [1, 2, 3]])
+    with pytest.raises(ValueError, match="two-dimensional"):
+        problems.smb_problem(_path("tet8cube0.smb"), 1, bc="vortex")
 
 
 @pytest.mark.parametrize("name,dim", [("squarevortex_small0.smb", 2), ("tet8cube0.smb", 3)])
