@@ -308,12 +308,9 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           // the reference evaluates F(q_j, q_k) with j > k and the direction
           // rows of dxidx at node j.  The IR flux is symmetric in its two
           // states (on the logavg series branch to the last bit: the sums
-          // commute and f changes sign only), so the plain kernel uses this
-          // node's own z as the left state and reads only the partner's from
-          // LDS.  The RK-stage kernel, whose epilogue holds more registers,
-          // reads both states from LDS in the reference's order (lanes of
-          // one wave hold both orientations; no branch, no select): keeping
-          // z in registers there spills.  The direction takes node j's metric.
+          // commute and f changes sign only), so this node's own z is the
+          // left state and only the partner's is read from LDS.  The
+          // direction takes node j's metric.
           const bool i_is_j = i > kk;
           // sum_d 2 S_d[i][kk] F_d is the IR flux in the one direction
           // n = sum_d 2 S_d[i][kk] dxidx[d, :] of node j (the flux is linear in
@@ -337,22 +334,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
               for (int d = 1; d < DIM; ++d) n[x] = n[x] + s2[d] * mk[x * DIM + d];
             }
           }
-          if constexpr (RK) {
-            const int jnode = i_is_j ? i : kk;
-            const int knode = i_is_j ? kk : i;
-            T zL[ND], zR[ND];
+          T zR[ND];
 #pragma unroll
-            for (int k = 0; k < ND; ++k) {
-              zL[k] = zpl[k * PL + me + jnode];
-              zR[k] = zpl[k * PL + me + knode];
-            }
-            ir_flux_z<DIM, 1>(phys, zL, zR, n, g);
-          } else {
-            T zR[ND];
-#pragma unroll
-            for (int k = 0; k < ND; ++k) zR[k] = zpl[k * PL + me + kk];
-            ir_flux_z<DIM, 1>(phys, zi, zR, n, g);
-          }
+          for (int k = 0; k < ND; ++k) zR[k] = zpl[k * PL + me + kk];
+          ir_flux_z<DIM, 1>(phys, zi, zR, n, g);
           // res_i -= 2 S[i][kk] F, and the partner gets -2 S[kk][i] F = +g
 #pragma unroll
           for (int k = 0; k < ND; ++k) ri[k] = ri[k] - g[k];
@@ -722,7 +707,9 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
       for (int c = 0; c < ND; ++c) {
         const double k = re(ri[c]) * minv;
-        const double qo = rk.stage == 0 ? re(qi[c]) : rk.q_old[base + c];
+        // stage 0: q_old is this stage's input, still staged in LDS (reading
+        // it there lets the entry registers qi die after the volume phase)
+        const double qo = rk.stage == 0 ? re(sm.q[el][i][c]) : rk.q_old[base + c];
         if (rk.stage == 3) {
           rk.q_next[base + c] = qo + rk.h6 * (rk.acc[base + c] + rk.wacc * k);
         } else {
