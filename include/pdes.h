@@ -190,6 +190,16 @@ int pdes_assemble_jacobian(pdes_ctx* ctx);
  * check of the element-wise Jacobian kernels (pdes_assemble_jacobian), and
  * the derivative path of the homotopy Jacobian. */
 int pdes_assemble_jacobian_coloring(pdes_ctx* ctx);
+/* The distance-2 element colouring that pdes_assemble_jacobian_coloring
+ * perturbs by, as the reference's mesh fields numColors, color and
+ * pertNeighborEls (docs/src/interfaces.md:535-570, used by
+ * calcJacobianSparse, src/jacobian/jacobian.jl:365-446): color[numEl]
+ * (0-based), *ncolors, and pert_neighbor_els[numEl][ncolors], the element of
+ * colour c among element i and its face neighbours, or -1.  Greedy in element
+ * order: element e takes the lowest colour not used by an element within two
+ * face-neighbour steps of it.  Any output pointer may be null (call once with
+ * pert_neighbor_els null to learn ncolors).  Needs pdes_jacobian_setup. */
+int pdes_get_coloring(pdes_ctx* ctx, int32_t* color, int32_t* ncolors, int32_t* pert_neighbor_els);
 int pdes_get_bsr_values(pdes_ctx* ctx, void* values_host);
 /* b = J x with the assembled BSR matrix (applyLinearOperator of PetscMatLO,
  * src/linearsolvers/lo_petscmat.jl), host vectors. */

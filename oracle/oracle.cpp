@@ -30,6 +30,8 @@
 #include <cstring>
 #include <vector>
 
+#include <omp.h>
+
 #include "../include/pdes.h"
 #include "euler_oracle.hpp"
 
@@ -59,6 +61,48 @@ template <class T> inline void acc(T& dst, const T& v, bool absmode) {
   if (absmode) dst += T(std::abs(v)); else dst += v;
 }
 
+// Which terms residual() adds (test selector, see oracle_set_parts): the
+// reference's own entropy-stability tests evaluate the terms one at a time
+// (test/euler/test_ESS.jl:445-578 with addVolumeIntegrals /
+// addBoundaryIntegrals off, test_functionals.jl:160-175).
+enum { PART_VOLUME = 1, PART_BOUNDARY = 2, PART_FACE = 4, PART_PENALTY = 8, PART_ALL = 15 };
+int g_parts = PART_ALL;
+int g_iface_only = -1;   // >= 0: the interface loop visits this interface only
+inline bool iface_skipped(int i) { return g_iface_only >= 0 && i != g_iface_only; }
+
+// Threads of the CPU timing baseline (oracle_set_threads).  With one thread
+// every loop runs in the reference's order; with more, elements are split
+// over OpenMP threads and faces run in groups whose faces touch disjoint
+// elements, so no residual entry is updated by two threads at once (the
+// order of the contributions to an entry then differs in the last bits).
+int g_threads = 1;
+
+// face index groups: one group 0..n-1 when serial, else a greedy colouring of
+// the faces by the elements they touch (els(i, a, b): elements of face i, b
+// = -1 for a boundary face)
+template <class Els>
+std::vector<std::vector<int>> face_groups(int n, int numEl, Els els) {
+  std::vector<std::vector<int>> groups;
+  if (g_threads <= 1) {
+    groups.emplace_back(n);
+    for (int i = 0; i < n; ++i) groups[0][i] = i;
+    return groups;
+  }
+  std::vector<uint64_t> used(numEl, 0);
+  for (int i = 0; i < n; ++i) {
+    int a, b;
+    els(i, a, b);
+    const uint64_t busy = used[a] | (b >= 0 ? used[b] : 0);
+    int c = 0;
+    while (busy >> c & 1) ++c;
+    if ((int)groups.size() <= c) groups.resize(c + 1);
+    groups[c].push_back(i);
+    used[a] |= uint64_t(1) << c;
+    if (b >= 0) used[b] |= uint64_t(1) << c;
+  }
+  return groups;
+}
+
 template <class T>
 int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
              const T* q, T* res, bool absmode) {
@@ -80,9 +124,11 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
   auto Q = [&](int d, int i, int j) { return s->Q[((size_t)d * Np + i) * Np + j]; };
 
   // ---- volume integrals
-  if (ph->volume_integral_type == 1) {
-    std::vector<T> F((size_t)Np * dim * nd);
+  if (!(g_parts & PART_VOLUME)) {
+  } else if (ph->volume_integral_type == 1) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int e = 0; e < numEl; ++e) {
+      T F[27 * 3 * 5];   // Np * dim * nd
       // getEulerFlux: flux_parametric[:, j, e, d] with nrm = dxidx[d, :, j, e]
       for (int j = 0; j < Np; ++j)
         for (int d = 0; d < dim; ++d) {
@@ -105,10 +151,11 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
     for (int d = 0; d < dim; ++d)
       for (int i = 0; i < Np; ++i)
         for (int j = 0; j < Np; ++j) S[((size_t)i * Np + j) * dim + d] = 0.5 * (Q(d, i, j) - Q(d, j, i));
-    T Fd[15];
-    double nrmD[9];
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
     for (int e = 0; e < numEl; ++e)
       for (int j = 0; j < Np; ++j) {
+        T Fd[15];
+        double nrmD[9];
         const T* qj = q + ((size_t)e * Np + j) * nd;
         for (int k = 0; k < j; ++k) {
           const T* qk = q + ((size_t)e * Np + k) * nd;
@@ -129,10 +176,20 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
   }
 
   // ---- boundary integrals (precompute_boundary_flux: getBCFluxes then boundaryintegrate!)
-  {
-    std::vector<T> qb((size_t)Nf * nd), fb((size_t)Nf * nd);
-    for (int bc = 0; bc < m->numBC; ++bc) {
-      for (int b = m->bndry_offsets[bc]; b < m->bndry_offsets[bc + 1]; ++b) {
+  if (g_parts & PART_BOUNDARY) {
+    const int nb = m->bndry_offsets[m->numBC] - m->bndry_offsets[0];
+    std::vector<int> bc_of(nb);
+    for (int bc = 0; bc < m->numBC; ++bc)
+      for (int b = m->bndry_offsets[bc]; b < m->bndry_offsets[bc + 1]; ++b) bc_of[b - m->bndry_offsets[0]] = bc;
+    auto groups = face_groups(nb, numEl, [&](int i, int& a, int& c) {
+      a = m->bndryfaces[2 * (i + m->bndry_offsets[0])];
+      c = -1;
+    });
+    for (const auto& grp : groups) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+      for (size_t gi = 0; gi < grp.size(); ++gi) {
+        const int b = grp[gi] + m->bndry_offsets[0], bc = bc_of[grp[gi]];
+        T qb[6 * 5], fb[6 * 5];   // Nf * nd
         int e = m->bndryfaces[2 * b], f = m->bndryfaces[2 * b + 1];
         // boundaryinterpolate!
         for (int kf = 0; kf < Nf; ++kf)
@@ -172,16 +229,24 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
     // faceElementIntegrals.jl:151-170): calcECFaceIntegral (:58-120) plus
     // calcEntropyPenaltyIntegral with the LF kernel (:199-290, :455-470)
     const int st_n = s->stencil;
-    T fluxD[15], wL[15 * 5], wR[15 * 5], wLi[5], wRi[5], qLi[5], qRi[5], qavg[5], dw[5], A0[25], flux[5];
     double nrmD[9];
     for (int a = 0; a < dim * dim; ++a) nrmD[a] = 0.0;
     for (int d = 0; d < dim; ++d) nrmD[d * dim + d] = 1.0;
-    for (int i = 0; i < m->numInterfaces; ++i) {
+    auto groups = face_groups(m->numInterfaces, numEl, [&](int i, int& a, int& c) {
+      a = m->interfaces[5 * i];
+      c = m->interfaces[5 * i + 1];
+    });
+    for (const auto& grp : groups) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+    for (size_t gi = 0; gi < grp.size(); ++gi) {
+      const int i = grp[gi];
+      if (iface_skipped(i)) continue;
+      T fluxD[15], wL[15 * 5], wR[15 * 5], wLi[5], wRi[5], qLi[5], qRi[5], qavg[5], dw[5], A0[25], flux[5];
       const int* it = m->interfaces + 5 * i;
       int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
       const double* nrm_xy = m->nrm_face + (size_t)i * Nf * dim;   // [k][d]
       // entropy conservative part
-      for (int ii = 0; ii < st_n; ++ii) {
+      for (int ii = 0; ii < st_n && (g_parts & PART_FACE); ++ii) {
         int p_i = s->perm[ii * nfaces + fL];
         const T* qi = q + ((size_t)eL * Np + p_i) * nd;
         for (int jj = 0; jj < st_n; ++jj) {
@@ -206,7 +271,7 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
         convertToIR(p, q + ((size_t)eL * Np + s->perm[ii * nfaces + fL]) * nd, &wL[ii * nd]);
         convertToIR(p, q + ((size_t)eR * Np + s->perm[ii * nfaces + fR]) * nd, &wR[ii * nd]);
       }
-      for (int k = 0; k < Nf; ++k) {
+      for (int k = 0; k < Nf && (g_parts & PART_PENALTY); ++k) {
         int nk = s->nbrperm[k * s->norient + o];
         const double* dir = nrm_xy + (size_t)k * dim;
         for (int c = 0; c < nd; ++c) { wLi[c] = 0.0; wRi[c] = 0.0; }
@@ -233,9 +298,18 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
         }
       }
     }
-  } else {
-    std::vector<T> qL((size_t)Nf * nd), qR((size_t)Nf * nd), ff((size_t)Nf * nd);
-    for (int i = 0; i < m->numInterfaces; ++i) {
+    }
+  } else if (g_parts & PART_FACE) {
+    auto groups = face_groups(m->numInterfaces, numEl, [&](int i, int& a, int& c) {
+      a = m->interfaces[5 * i];
+      c = m->interfaces[5 * i + 1];
+    });
+    for (const auto& grp : groups) {
+#pragma omp parallel for num_threads(g_threads) schedule(static) if (g_threads > 1)
+    for (size_t gi = 0; gi < grp.size(); ++gi) {
+      const int i = grp[gi];
+      if (iface_skipped(i)) continue;
+      T qL[6 * 5], qR[6 * 5], ff[6 * 5];   // Nf * nd
       const int* it = m->interfaces + 5 * i;
       int eL = it[0], eR = it[1], fL = it[2], fR = it[3], o = it[4];
       // interiorfaceinterpolate!
@@ -270,6 +344,7 @@ int residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_de
           }
         }
       }
+    }
     }
   }
   return PDES_OK;
@@ -396,6 +471,22 @@ std::vector<std::vector<int>> element_neighbors(const pdes_mesh_desc* m) {
 
 extern "C" {
 
+// Test selector: the terms the residual adds from now on (bits: 1 volume,
+// 2 boundary, 4 interface flux / entropy-conservative face integral,
+// 8 entropy penalty of the ESLF face integral; 15 = the full residual) and
+// the one interface the interface loop visits (-1: all).
+void oracle_set_parts(int parts, int iface_only) {
+  g_parts = parts;
+  g_iface_only = iface_only;
+}
+
+// Threads of residual evaluations from now on (1: serial, the reference's
+// summation order); returns the number of host cores OpenMP sees.
+int oracle_set_threads(int threads) {
+  g_threads = threads < 1 ? 1 : threads;
+  return omp_get_num_procs();
+}
+
 int oracle_residual(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_phys_desc* ph,
                     const double* q, double* res) {
   return residual<double>(m, s, ph, q, res, false);
@@ -509,6 +600,23 @@ int oracle_color_elements(const pdes_mesh_desc* m, int32_t* color, int32_t* ncol
     if (c == nc) ++nc;
   }
   *ncolors = nc;
+  return PDES_OK;
+}
+
+// pertNeighborEls (docs/src/interfaces.md:566-569): [numEl][numColors], the
+// element whose perturbation reaches element i when colour c is perturbed
+// (i itself or one of its face neighbours), -1 for none.
+int oracle_pert_neighbor_els(const pdes_mesh_desc* m, const int32_t* color, int32_t ncolors, int32_t* pert) {
+  auto nb = element_neighbors(m);
+  std::fill(pert, pert + (size_t)m->numEl * ncolors, -1);
+  for (int e = 0; e < m->numEl; ++e) {
+    pert[(size_t)e * ncolors + color[e]] = e;
+    for (int a : nb[e]) {
+      int32_t& slot = pert[(size_t)e * ncolors + color[a]];
+      if (slot >= 0 && slot != a) return PDES_ERR_ARG;   // not a distance-2 colouring
+      slot = a;
+    }
+  }
   return PDES_OK;
 }
 

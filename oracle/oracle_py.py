@@ -118,6 +118,16 @@ def color_elements(b):
     return color, nc.value
 
 
+def pert_neighbor_els(b, color, ncolors):
+    color = np.ascontiguousarray(color, dtype=np.int32)
+    out = np.zeros((b.mesh.numEl, ncolors), dtype=np.int32)
+    L = lib()
+    L.oracle_pert_neighbor_els.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    _check(L.oracle_pert_neighbor_els(C.addressof(b.mesh), color.ctypes.data, ncolors, out.ctypes.data),
+           "pertNeighborEls")
+    return out
+
+
 def jacobian_cs(b, q, rowptr, colind):
     nd = b.mesh.dim + 2
     q = np.ascontiguousarray(q, dtype=np.float64)
@@ -210,4 +220,8 @@ def functional(m, op, ph, q, bcnums):
     else:
         lift = -np.sin(aoa) * force[0] + np.cos(aoa) * force[2]
         drag = np.cos(aoa) * force[0] + np.sin(aoa) * force[2]
-    return {"force": force, "lift": lift, "drag": drag, "entropy": ent, "mass": mass}
+    # LiftCoefficient / DragCoefficient (boundary_functional.jl:180-191):
+    # divided by the dynamic pressure 0.5 rho_free Ma^2
+    qdyn = 0.5 * ph.rho_free * ph.Ma * ph.Ma
+    return {"force": force, "lift": lift, "drag": drag, "entropy": ent, "mass": mass,
+            "lift_coef": lift / qdyn, "drag_coef": drag / qdyn}

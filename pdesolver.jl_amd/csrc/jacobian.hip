@@ -405,6 +405,7 @@ static int coloring_setup(pdes_ctx* c) {
     color[e] = col;
     ncolor = std::max(ncolor, col + 1);
   }
+  J->h_color = color;
   J->color_ptr.assign(ncolor + 1, 0);
   for (int e = 0; e < nel; ++e) J->color_ptr[color[e] + 1]++;
   for (int k = 0; k < ncolor; ++k) J->color_ptr[k + 1] += J->color_ptr[k];
@@ -529,6 +530,29 @@ int pdes_assemble_jacobian_coloring(pdes_ctx* c) {
   int st = assemble_coloring(c);
   if (st) return st;
   PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_get_coloring(pdes_ctx* c, int32_t* color, int32_t* ncolors, int32_t* pert_neighbor_els) {
+  PDES_CHECK(c && c->jac, PDES_ERR_ARG, "call pdes_jacobian_setup first");
+  int st = coloring_setup(c);
+  if (st) return st;
+  const pdes_jacobian* J = c->jac;
+  const int nel = c->numEl, nf = c->nfaces, nc = (int)J->color_ptr.size() - 1;
+  if (ncolors) *ncolors = nc;
+  if (color) std::copy(J->h_color.begin(), J->h_color.end(), color);
+  if (pert_neighbor_els) {
+    // pertNeighborEls[i, c]: the element of colour c among i and its face
+    // neighbours (at most one, the colouring being distance-2), or -1
+    std::fill(pert_neighbor_els, pert_neighbor_els + (size_t)nel * nc, -1);
+    for (int e = 0; e < nel; ++e) {
+      pert_neighbor_els[(size_t)e * nc + J->h_color[e]] = e;
+      for (int f = 0; f < nf; ++f) {
+        const int a = c->h_elface[(size_t)e * nf + f].x;
+        if (a >= 0) pert_neighbor_els[(size_t)e * nc + J->h_color[a]] = a;
+      }
+    }
+  }
   return PDES_OK;
 }
 

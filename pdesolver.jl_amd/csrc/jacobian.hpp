@@ -18,6 +18,7 @@ struct pdes_jacobian {
   // complex-step coloring assembly (calcJacobianSparse): elements grouped by
   // a distance-2 colouring, the seed and product vectors
   std::vector<int64_t> color_ptr;
+  std::vector<int32_t> h_color;   // colour of each element
   int32_t* d_color_el = nullptr;
   double* d_seed = nullptr;
   double* d_jv = nullptr;

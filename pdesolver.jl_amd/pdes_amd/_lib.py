@@ -46,6 +46,7 @@ _SIGS = {
     "pdes_get_bsr_pattern": [_vp, _vp, _vp],
     "pdes_assemble_jacobian": [_vp],
     "pdes_assemble_jacobian_coloring": [_vp],
+    "pdes_get_coloring": [_vp, _vp, _i32p, _vp],
     "pdes_get_bsr_values": [_vp, _vp],
     "pdes_bsr_matvec": [_vp, _vp, _vp],
     "pdes_pc_setup": [_vp, C.c_int],

@@ -37,6 +37,20 @@ def calcJacobianSparse(mesh, sbp, eqn: EulerData, opts=None, fp64_blocks: bool =
     check(lib().pdes_assemble_jacobian_coloring(eqn.ctx))
 
 
+def getColoring(eqn: EulerData, fp64_blocks: bool = False):
+    """The mesh colouring fields of the reference (numColors, color,
+    pertNeighborEls; docs/src/interfaces.md:535-570) as the device colouring
+    assembly uses them: (color[numEl], numColors, pertNeighborEls[numEl,
+    numColors]), 0-based, -1 where no element of that colour is adjacent."""
+    check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))
+    nc = C.c_int32()
+    check(lib().pdes_get_coloring(eqn.ctx, None, C.byref(nc), None))
+    color = np.zeros(eqn.mesh.numEl, dtype=np.int32)
+    pert = np.zeros((eqn.mesh.numEl, nc.value), dtype=np.int32)
+    check(lib().pdes_get_coloring(eqn.ctx, color.ctypes.data, C.byref(nc), pert.ctypes.data))
+    return color, nc.value, pert
+
+
 def getSparsityPattern(eqn: EulerData, fp64_blocks: bool = False):
     """(rowptr, colind) of the node-block BSR matrix (0-based, sorted)."""
     check(lib().pdes_jacobian_setup(eqn.ctx, int(fp64_blocks)))

@@ -29,3 +29,35 @@ def assert_parity(got, ref, scale, rtol=1e-12, what=""):
     worst = np.max(err / bound)
     assert worst <= 1.0, f"{what}: max err/bound = {worst:.3g}, max abs err {err.max():.3g}"
     return worst
+
+
+def boundary_flux_sum(oracle, m, op, ph, q):
+    """sum over boundary faces and face nodes of B_k F_Roe(R q, q_bc; n_k), the
+    boundary integral of the oracle's Roe flux against the BC state of each
+    group (vortex or freestream), per conserved variable."""
+    import ctypes as C
+    f = op.face
+    nd = m.dim + 2
+    pd = ph.desc()
+    roe = oracle.lib().oracle_roe
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    tot = np.zeros(nd)
+    Fk = np.zeros(nd)
+    for bc in range(m.numBC):
+        lo, hi = m.bndry_offsets[bc], m.bndry_offsets[bc + 1]
+        if hi == lo:
+            continue
+        if ph.bc_type[bc] == abi.PDES_BC_ISENTROPIC_VORTEX:
+            qg_all = problems.vortex_exact(m.coords_bndry[lo:hi].reshape(-1, m.dim), ph).reshape(hi - lo, -1, nd)
+        else:
+            qg_all = np.broadcast_to(problems.freestream((), m.dim, ph), (hi - lo, f.numnodes, nd))
+        for bi in range(lo, hi):
+            e, fc = m.bndryfaces[bi]
+            qb = f.interp.T @ q[e][f.perm[:, fc]]
+            for k in range(f.numnodes):
+                qL = np.ascontiguousarray(qb[k])
+                qR = np.ascontiguousarray(qg_all[bi - lo, k])
+                n = np.ascontiguousarray(m.nrm_bndry[bi, k])
+                roe(m.dim, C.byref(pd), dp(qL), dp(qR), dp(n), dp(Fk))
+                tot += f.wface[k] * Fk
+    return tot

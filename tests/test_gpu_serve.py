@@ -76,3 +76,75 @@ def test_footprint_estimate_covers_the_allocations():
     used = free0 - free1
     print("estimate", est, "measured", used, "ratio", used / est)
     assert 0.6 * est <= used <= 1.15 * est
+
+
+def _minv(m, op):
+    return np.repeat((1.0 / (op.w[None, :] / m.jac))[:, :, None], m.dim + 2, axis=2)
+
+
+def test_served_results_match_oracle_direct_solves(oracle):
+    """Every served result (a mix of 2D vortex and 3D freestream requests at
+    sizes the oracle solves in seconds, p in {1, 2}, Mach and angle of attack
+    drawn as in the serving workload) against the oracle's Newton with a
+    sparse direct solve: both reach ||R|| <= 1e-10 and the states agree to
+    1e-9 of their magnitude."""
+    reqs = _requests()
+    srv = SolveServer(4 * max(estimate_bytes(m, op, ph, OPTS) for (m, op, ph, q0) in reqs), lanes=2, opts=OPTS)
+    tickets = [srv.submit(m, op, ph, q0) for (m, op, ph, q0) in reqs]
+    for t in tickets:
+        assert t.wait(120.0), "request did not finish"
+    srv.shutdown()
+    for i, (t, (m, op, ph, q0)) in enumerate(zip(tickets, reqs)):
+        print(f"request {i}: dim {m.dim} p={op.degree} numEl {m.numEl}", flush=True)
+        assert t.error is None, t.error
+        assert t.report.converged == 1 and t.report.res_norm <= 1e-10
+        b = abi.make_descriptors(m, op, ph)
+        q_ref, norms = oracle.newton_direct(b, q0, _minv(m, op), res_abstol=1e-10)
+        assert norms[-1] < 1e-10
+        assert np.abs(t.q - q_ref).max() <= 1e-9 * np.abs(q_ref).max()
+
+
+def _full_size_requests(count, seed):
+    """Requests at the serving workload's sizes: 70% 2D vortex grids with
+    edges in 32..128, 30% 3D freestream cubes with 8..24 cells per edge,
+    p in {1, 2}, M in [0.3, 0.8], angle of attack in [-5, 5] degrees."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        p = int(rng.integers(1, 3))
+        Ma = float(rng.uniform(0.3, 0.8))
+        aoa = float(rng.uniform(-5.0, 5.0))
+        if rng.random() < 0.7:
+            n = int(rng.integers(32, 129))
+            m, op, ph = problems.make_problem("vortex", n, p, "omega", "roe", Ma=Ma, aoa_deg=aoa, vortex_Min=Ma)
+            q0 = problems.vortex_noise_state(m, ph, seed=seed * 100 + i)
+        else:
+            n = int(rng.integers(8, 25))
+            m, op, ph = problems.make_problem("cube", n, p, "omega", "roe", Ma=Ma, aoa_deg=aoa)
+            q0 = problems.freestream_bump_state(m, ph, seed=seed * 100 + i)
+        out.append((m, op, ph, q0))
+    return out
+
+
+def test_served_full_size_requests_converge_and_equal_solo():
+    """At the serving workload's full request sizes: every served request
+    converges to ||R|| <= 1e-10 and equals, bit for bit, the same request
+    solved alone."""
+    reqs = _full_size_requests(6, 41)
+    budget = 3 * max(estimate_bytes(m, op, ph, OPTS) for (m, op, ph, q0) in reqs)
+    srv = SolveServer(budget, lanes=2, opts=OPTS)
+    tickets = [srv.submit(m, op, ph, q0) for (m, op, ph, q0) in reqs]
+    for i, t in enumerate(tickets):
+        assert t.wait(300.0), "request did not finish"
+        print(f"served {i}: numEl {reqs[i][0].numEl} p={reqs[i][1].degree} latency {t.latency:.2f} s", flush=True)
+    srv.shutdown()
+    for i, (t, (m, op, ph, q0)) in enumerate(zip(tickets, reqs)):
+        assert t.error is None, t.error
+        assert t.report.converged == 1 and t.report.res_norm <= 1e-10
+        solo = BatchSolver([(m, op, ph)])
+        solo.set_state(0, q0)
+        r = solo.solve(OPTS)[0]
+        print(f"solo {i}: iterations {r.iterations} (served {t.report.iterations})", flush=True)
+        assert r.iterations == t.report.iterations
+        assert np.array_equal(solo.get_state(0), t.q), f"request {i} differs from its solo solve"
+        solo.free()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from pdes_amd import abi, problems
-from helpers import setup
+from helpers import boundary_flux_sum, setup
 
 PH = abi.Physics().desc()
 
@@ -269,31 +269,8 @@ def test_conservation(oracle, kind, n, p, opk, flux):
     # contribution via a residual of a state with zero boundary flux is not
     # available, so compare the total against the oracle boundary flux sum
     tot = r.sum(axis=(0, 1))
-    bsum = _boundary_flux_sum(oracle, m, op, ph, q)
+    bsum = boundary_flux_sum(oracle, m, op, ph, q)
     assert np.abs(tot + bsum).max() < 1e-12 * (1 + np.abs(bsum).max())
-
-
-def _boundary_flux_sum(oracle, m, op, ph, q):
-    f = op.face
-    nd = m.dim + 2
-    tot = np.zeros(nd)
-    for bc in range(m.numBC):
-        for bi in range(m.bndry_offsets[bc], m.bndry_offsets[bc + 1]):
-            e, fc = m.bndryfaces[bi]
-            qe = q[e]
-            qb = f.interp.T @ qe[f.perm[:, fc]]
-            for k in range(f.numnodes):
-                if ph.bc_type[bc] == abi.PDES_BC_ISENTROPIC_VORTEX:
-                    qg = problems.vortex_exact(m.coords_bndry[bi, k][None, :], ph)[0]
-                else:
-                    qg = problems.freestream((), m.dim, ph)
-                Fk = roe(oracle, qb[k], qg, m.nrm_bndry[bi, k]) if m.dim == 2 else None
-                if m.dim == 3:
-                    Fk = np.zeros(nd)
-                    oracle.lib().oracle_roe(3, C.byref(PH), _p(_arr(qb[k])), _p(_arr(qg)),
-                                            _p(_arr(m.nrm_bndry[bi, k])), _p(Fk))
-                tot += f.wface[k] * Fk
-    return tot
 
 
 def test_absvalue_kink_semantics(oracle):
@@ -411,3 +388,22 @@ def test_implicit_euler_globalization_limits(oracle):
     qb, nb = oracle.newton_direct(b, q0, minv, res_abstol=1e-11, euler_tau=1.0, jac=m.jac)
     assert na[-1] < 1e-11 and nb[-1] < 1e-11 and len(nb) >= len(na)
     assert np.abs(qa - qb).max() <= 1e-10 * np.abs(qa).max()
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", [("vortex", 6, 2, "diage", "roe"), ("vortex", 6, 2, "diage", "ir"),
+                                              ("vortex", 4, 2, "omega", "es"), ("cube", 2, 2, "omega", "roe")])
+def test_threaded_oracle_matches_serial(oracle, kind, n, p, opk, flux):
+    """The threaded residual of the CPU timing baseline (elements over
+    threads, faces in element-disjoint groups) equals the serial one up to
+    the order of the contributions to each entry."""
+    m, op, ph, b = setup(kind, n, p, opk, flux)
+    q = problems.vortex_noise_state(m, ph) if kind == "vortex" else problems.freestream_bump_state(m, ph)
+    ref = oracle.residual(b, q)
+    scale = oracle.residual_scale(b, q)
+    L = oracle.lib()
+    L.oracle_set_threads(4)
+    try:
+        got = oracle.residual(b, q)
+    finally:
+        L.oracle_set_threads(1)
+    assert np.all(np.abs(got - ref) <= 1e-14 * (np.abs(ref) + scale))
