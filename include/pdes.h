@@ -160,6 +160,20 @@ int pdes_copy_bandwidth(int64_t nbytes, int iters, double* avg_ms);
 /* Free and total device memory in bytes (hipMemGetInfo), for footprint checks. */
 int pdes_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 int pdes_rk4(pdes_ctx* ctx, int nsteps, double h, double* elapsed_ms);
+/* No counterpart in the reference: pdes_rk4 for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Runs nsteps RK4 steps on the caller's device state
+ * q_dev (numDof doubles, updated in place) with the context's operator, mesh
+ * and launch variants, keeping the stage accumulator, the two stage states and
+ * the positivity flag in a fixed per-GPU device array of the library (48 MiB,
+ * held until the process ends) instead of context buffers: the call allocates
+ * nothing.  Contexts with numDof above 2^21 (2,097,152) are rejected with
+ * PDES_ERR_ARG; batch contexts with PDES_ERR_UNSUPPORTED.  Like pdes_rk4 it
+ * waits for the context's stream and returns PDES_ERR_PHYS on a non-positive
+ * density or pressure.  Every call on one GPU shares that GPU's array, so
+ * calls that overlap in time on one GPU (from several threads or streams)
+ * race and must be serialised by the caller. */
+int pdes_rk4_noalloc(pdes_ctx* ctx, double* q_dev, int nsteps, double h);
 /* b_dev = (dR/dq) v_dev at the device state, device vectors of numDof doubles
  * (the device-pointer form of pdes_jv_complex_step). */
 int pdes_jv_device(pdes_ctx* ctx, const double* v_dev, double* b_dev);

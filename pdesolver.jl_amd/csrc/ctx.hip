@@ -69,6 +69,11 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   PDES_CHECK(m->numEl > 0 && s->Np > 0 && s->Nf > 0, PDES_ERR_ARG, "empty mesh or operator");
   pdes_ctx* c = new pdes_ctx();
   c->use_graphs = std::getenv("PDES_NO_GRAPHS") == nullptr;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    set_error("hipGetDevice failed");
+    return PDES_ERR_HIP;
+  }
   c->dim = m->dim;
   c->nd = m->dim + 2;
   c->nfaces = m->dim + 1;

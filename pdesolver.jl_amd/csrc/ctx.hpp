@@ -17,6 +17,7 @@ struct pdes_ctx {
   pdes::PhysConst pc{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  int device = 0;             // HIP device current at creation (owns the stream and buffers)
   // device tables
   double* d_dxidx = nullptr;
   pdes::ElFace* d_elface = nullptr;
@@ -93,8 +94,9 @@ struct pdes_ctx {
 
 namespace pdes {
 // residual launch (value run, or directional derivative when v != nullptr)
+// (flag, when set, replaces the context's positivity flag word)
 int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv,
-                    bool check, const RKStage* rk = nullptr);
+                    bool check, const RKStage* rk = nullptr, int32_t* flag = nullptr);
 // deterministic sum of x_i^2 * w_i (w may be null) into *out_dev
 int launch_wnorm2(pdes_ctx* c, const double* x, const double* w, int64_t n, double* out_dev);
 int sync_check_flag(pdes_ctx* c);

@@ -6,8 +6,8 @@
 namespace pdes {
 
 int launch_residual(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, bool check,
-                    const RKStage* rk) {
-  int32_t* flag = check ? c->d_flag : nullptr;
+                    const RKStage* rk, int32_t* flag_dev) {
+  int32_t* flag = check ? (flag_dev ? flag_dev : c->d_flag) : nullptr;
 #define PDES_DISPATCH(D, NP, NF, ST, DE)                                                        \
   if (c->dim == D && c->Np == NP && c->Nf == NF && c->stencil == ST && (c->diag_e != 0) == DE) { \
     using Op = OpShape<D, NP, NF, ST, DE>;                                                      \

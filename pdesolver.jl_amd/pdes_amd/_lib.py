@@ -41,6 +41,7 @@ _SIGS = {
     "pdes_autotune": [_vp, C.c_int, _i32p],
     "pdes_set_launch_variant": [_vp, C.c_int, C.c_int],
     "pdes_rk4": [_vp, C.c_int, C.c_double, _dp],
+    "pdes_rk4_noalloc": [_vp, _vp, C.c_int, C.c_double],
     "pdes_jacobian_setup": [_vp, C.c_int],
     "pdes_jacobian_nnzb": [_vp, _i64p, _i64p],
     "pdes_get_bsr_pattern": [_vp, _vp, _vp],
