@@ -216,6 +216,34 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
     }
   }
   if (!st) st = upload(&c->d_elface, c->h_elface.data(), c->h_elface.size(), sm);
+  if (s->diag_e) {
+    // face slot records (FaceSlot): the neighbour's face node as a global node
+    // index and the normal row, so the residual's face phase does not walk
+    // record -> nbrperm -> perm -> state on the device
+    std::vector<FaceSlot> slots((size_t)m->numEl * nfaces * Nf);
+    for (int e = 0; e < m->numEl; ++e)
+      for (int f = 0; f < nfaces; ++f) {
+        const ElFace& r = c->h_elface[(size_t)e * nfaces + f];
+        for (int k = 0; k < Nf; ++k) {
+          FaceSlot& sl = slots[((size_t)e * nfaces + f) * Nf + k];
+          if (r.x < 0) {
+            const int64_t row = (int64_t)r.y * Nf + k;
+            sl = FaceSlot{(int32_t)(-1 - row), slot_info(row, k, 0)};
+          } else {
+            const int fN = r.z & 0xff, o = (r.z >> 8) & 0xff, side = (r.z >> 16) & 1;
+            const int kR = s->nbrperm[k * s->norient + o];
+            const int nodeN = s->perm[(side == 0 ? kR : k) * nfaces + fN];
+            sl = FaceSlot{(int32_t)((int64_t)r.x * Np + nodeN), slot_info((int64_t)r.y * Nf + k, side == 0 ? k : kR, side)};
+          }
+        }
+      }
+    if (!((int64_t)m->numEl * Np < ((int64_t)1 << 31) && (int64_t)std::max(nI, nB) * Nf < ((int64_t)1 << 27))) {
+      set_error("mesh too large for the face slot records");
+      st = PDES_ERR_ARG;
+    }
+    if (!st) st = upload(&c->d_slot, slots.data(), slots.size(), sm);
+    if (!st) st = (hipStreamSynchronize(sm) == hipSuccess) ? PDES_OK : PDES_ERR_HIP;
+  }
   if (!st) st = upload(&c->d_nrm_face, m->nrm_face, nI * Nf * dim, sm);
   if (!st) st = upload(&c->d_nrm_bndry, m->nrm_bndry, nB * Nf * dim, sm);
   if (!st) st = upload(&c->d_coords_bndry, m->coords_bndry, nB * Nf * dim, sm);
@@ -303,7 +331,7 @@ int pdes_ctx_destroy(pdes_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   jacobian_free(c);
   precond_free(c);
-  void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q, c->d_S2c,
+  void* ptrs[] = {c->d_dxidx, c->d_elface, c->d_slot, c->d_nrm_face, c->d_nrm_bndry, c->d_coords_bndry, c->d_Q, c->d_S2c,
                   c->d_interp, c->d_perm, c->d_nbrperm, c->d_nodepos, c->d_wface, c->d_Minv, c->d_Minv_node, c->d_q,
                   c->d_res, c->d_work, c->d_work2, c->d_partial, c->d_scalar, c->d_flag,
                   c->d_el_list, c->d_el_inst, c->d_inst_phys, c->d_ptc_diag, c->d_ptc_scale,

@@ -21,6 +21,7 @@ struct pdes_ctx {
   // device tables
   double* d_dxidx = nullptr;
   pdes::ElFace* d_elface = nullptr;
+  pdes::FaceSlot* d_slot = nullptr;   // diagonal-E face slots [e][f * Nf + k]
   double* d_nrm_face = nullptr;
   double* d_nrm_bndry = nullptr;
   double* d_coords_bndry = nullptr;
@@ -82,7 +83,7 @@ struct pdes_ctx {
 
   pdes::DevMesh devmesh() const {
     return pdes::DevMesh{numEl, metric_per_node, nlist >= 0 ? d_el_list : nullptr, nlist, d_el_inst, d_inst_phys,
-                         d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry, d_qbc};
+                         d_dxidx, d_elface, d_nrm_face, d_nrm_bndry, d_coords_bndry, d_qbc, d_slot};
   }
   // elements processed by a launch (the work list when one is set)
   int64_t active_count() const { return nlist >= 0 ? nlist : numEl; }

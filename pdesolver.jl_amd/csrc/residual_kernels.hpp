@@ -27,6 +27,19 @@ namespace pdes {
 //   w: BC functor of a boundary face (PDES_BC_*) | instance << 8 (packed batches)
 struct ElFace { int32_t x, y, z, w; };
 
+// diagonal-E face slot s = f * Nf + k of an element (k in elementL's face-node
+// numbering), built once per context so that the face phase is one dependent
+// load away from the neighbour's state:
+//   x: global node index (neighbour element * Np + node) of the neighbour's
+//      face node, or -1 - row for a boundary face;
+//   y: row << 4 | m << 1 | side, with row = interface * Nf + k (nrm_face) or
+//      boundary face * Nf + k (nrm_bndry, boundary states), m the face node in
+//      this element's numbering and side 1 when this element is elementR.
+struct FaceSlot { int32_t x, y; };
+__host__ __device__ inline int32_t slot_info(int64_t row, int m, int side) {
+  return (int32_t)((row << 4) | (m << 1) | side);
+}
+
 struct PhysConst;
 
 struct DevMesh {
@@ -46,6 +59,7 @@ struct DevMesh {
   const double* __restrict__ nrm_bndry;   // [b][k][dim]
   const double* __restrict__ coords_bndry;// [b][k][dim]
   const double* __restrict__ qbc;         // [b][k][ndof] boundary states (functions of the coordinates only)
+  const FaceSlot* __restrict__ slot;      // [e][f * Nf + k] (diagonal-E operators)
 };
 
 struct DevSBP {
@@ -110,6 +124,26 @@ template <> struct Io<dual<1>> {
   }
 };
 
+// state and normal of a diagonal-E face slot: the neighbour's face-node state
+// (or the boundary state, a constant) and the scaled normal of the face node
+template <int DIM, class T>
+__device__ __forceinline__ void load_slot(const DevMesh& m, const double* qin, const double* vin, FaceSlot s, T* qn,
+                                          double* nrm) {
+  const int64_t row = s.y >> 4;
+  const double* nr;
+  if (s.x >= 0) {
+#pragma unroll
+    for (int c = 0; c < DIM + 2; ++c) qn[c] = Io<T>::load(qin, vin, (int64_t)s.x * (DIM + 2) + c);
+    nr = m.nrm_face + row * DIM;
+  } else {
+#pragma unroll
+    for (int c = 0; c < DIM + 2; ++c) qn[c] = T(m.qbc[row * (DIM + 2) + c]);
+    nr = m.nrm_bndry + row * DIM;
+  }
+#pragma unroll
+  for (int x = 0; x < DIM; ++x) nrm[x] = nr[x];
+}
+
 enum { VOL_WEAK = 1, VOL_SPLIT = 2 };
 
 // RK4 stage epilogue (rk4.jl:238-319 with pde_post_func :446-460): instead of
@@ -171,21 +205,6 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   __shared__ Smem sm;
 
   const int tid = threadIdx.x;
-  // stage operator tables
-  if constexpr (VOL == VOL_SPLIT) {
-    // the split form only needs the antisymmetric part Q - Q^T = 2 S, and
-    // only in the order the pair rounds read it: entry [d][dd][a] holds
-    // 2 S_d[a][a+dd mod NP], so the lanes of one round read consecutive words
-    // (the table is built once per context, sbp.S2c)
-    for (int x = tid; x < DIM * (NP / 2 + 1) * NP; x += blockDim.x) sm.Q[x] = sbp.S2c[x];
-  } else {
-    for (int x = tid; x < DIM * NP * NP; x += blockDim.x) sm.Q[x] = sbp.Q[x];
-  }
-  for (int x = tid; x < ST * NF; x += blockDim.x) (&sm.interp[0][0])[x] = sbp.interp[x];
-  for (int x = tid; x < ST * NFACES; x += blockDim.x) (&sm.perm[0][0])[x] = sbp.perm[x];
-  for (int x = tid; x < NP * NFACES; x += blockDim.x) (&sm.nodepos[0][0])[x] = sbp.nodepos[x];
-  for (int x = tid; x < NF; x += blockDim.x) sm.wface[x] = sbp.wface[x];
-
   const int el = tid / NP;
   const int i = tid - el * NP;
   // LDS planes: value c of node n of this block's element el at
@@ -199,6 +218,20 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   const int64_t eslot = (int64_t)blockIdx.x * EPB + el;
   const bool active = (el < EPB) && (eslot < launch_count(mesh));
   const int64_t e = active ? element_of(mesh, eslot) : 0;
+
+  // diagonal-E: the slot record of this thread's first face slot (f, k) =
+  // (i / NF, i % NF), then the neighbour's face-node state (or the boundary
+  // state) and the normal it names, are issued first, so that these two
+  // dependent loads overlap with the element's own loads and the volume phase
+  FaceSlot slot0{0, 0};
+  T qn0[ND];
+  double nrm0[DIM];
+  if constexpr (Op::DIAGE) {
+    if (active && i < NFACES * NF) {
+      slot0 = mesh.slot[e * (NFACES * NF) + i];
+      load_slot<DIM, T>(mesh, qin, vin, slot0, qn0, nrm0);
+    }
+  }
 
   T qi[ND];
   if (active) {
@@ -219,32 +252,21 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
       for (int d = 0; d < DIM; ++d) dxi[x][d] = metric<DIM, NP>(mesh, e, i)[x * DIM + d];
   }
-
-  // diagonal-E: the record, normal and neighbour face-node state of this
-  // thread's first face slot are loaded before the volume phase, so the
-  // dependent global loads (record -> neighbour) overlap with it.
-  ElFace rec0{-1, 0, 0, 0};
-  T qn0[ND];
-  double nrm0[DIM];
-  if constexpr (Op::DIAGE) {
-    if (active && i < NFACES * NF) {
-      const int f = i / NF, k = i - f * NF;
-      rec0 = mesh.elface[e * NFACES + f];
-      const double* nr;
-      if (rec0.x >= 0) {
-        const int fN = rec0.z & 0xff, o = (rec0.z >> 8) & 0xff, side = (rec0.z >> 16) & 1;
-        const int kR = sbp.nbrperm[k * sbp.norient + o];
-        const int nodeN = sbp.perm[(side == 0 ? kR : k) * NFACES + fN];
-#pragma unroll
-        for (int c = 0; c < ND; ++c) qn0[c] = Io<T>::load(qin, vin, ((int64_t)rec0.x * NP + nodeN) * ND + c);
-        nr = mesh.nrm_face + ((int64_t)rec0.y * NF + k) * DIM;
-      } else {
-        nr = mesh.nrm_bndry + ((int64_t)rec0.y * NF + k) * DIM;
-      }
-#pragma unroll
-      for (int x = 0; x < DIM; ++x) nrm0[x] = nr[x];
-    }
+  // stage operator tables
+  if constexpr (VOL == VOL_SPLIT) {
+    // the split form only needs the antisymmetric part Q - Q^T = 2 S, and
+    // only in the order the pair rounds read it: entry [d][dd][a] holds
+    // 2 S_d[a][a+dd mod NP], so the lanes of one round read consecutive words
+    // (the table is built once per context, sbp.S2c)
+    for (int x = tid; x < DIM * (NP / 2 + 1) * NP; x += blockDim.x) sm.Q[x] = sbp.S2c[x];
+  } else {
+    for (int x = tid; x < DIM * NP * NP; x += blockDim.x) sm.Q[x] = sbp.Q[x];
   }
+  for (int x = tid; x < ST * NF; x += blockDim.x) (&sm.interp[0][0])[x] = sbp.interp[x];
+  for (int x = tid; x < ST * NFACES; x += blockDim.x) (&sm.perm[0][0])[x] = sbp.perm[x];
+  for (int x = tid; x < NP * NFACES; x += blockDim.x) (&sm.nodepos[0][0])[x] = sbp.nodepos[x];
+  for (int x = tid; x < NF; x += blockDim.x) sm.wface[x] = sbp.wface[x];
+
 
   T ri[ND];
 #pragma unroll
@@ -362,52 +384,36 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     if (active) {
       for (int slot = i; slot < NFACES * NF; slot += NP) {
         const int f = slot / NF;
-        const int k = slot - f * NF;
-        ElFace rec;
+        FaceSlot sr;
         T qn[ND];
         double nrm[DIM];
         if (slot == i) {
-          rec = rec0;
+          sr = slot0;
 #pragma unroll
           for (int c = 0; c < ND; ++c) qn[c] = qn0[c];
 #pragma unroll
           for (int x = 0; x < DIM; ++x) nrm[x] = nrm0[x];
         } else {
-          rec = mesh.elface[e * NFACES + f];
-          const double* nr;
-          if (rec.x >= 0) {
-            const int fN = rec.z & 0xff, o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
-            const int kR = sbp.nbrperm[k * sbp.norient + o];
-            const int nodeN = sm.perm[side == 0 ? kR : k][fN];
-#pragma unroll
-            for (int c = 0; c < ND; ++c) qn[c] = Io<T>::load(qin, vin, ((int64_t)rec.x * NP + nodeN) * ND + c);
-            nr = mesh.nrm_face + ((int64_t)rec.y * NF + k) * DIM;
-          } else {
-            nr = mesh.nrm_bndry + ((int64_t)rec.y * NF + k) * DIM;
-          }
-#pragma unroll
-          for (int x = 0; x < DIM; ++x) nrm[x] = nr[x];
+          sr = mesh.slot[e * (NFACES * NF) + slot];
+          load_slot<DIM, T>(mesh, qin, vin, sr, qn, nrm);
         }
+        // m: the face node in this element's numbering (k as elementL, the
+        // neighbour-permuted kR as elementR)
+        const int m = (sr.y >> 1) & 7, side = sr.y & 1;
+        const int node = sm.perm[m][f];
         T qL[ND], qR[ND], Fk[ND];
-        int m;
         double sign;
-        if (rec.x < 0) {
-          m = k;
+        if (sr.x < 0) {
+          // boundary: own face node against the boundary state, Roe flux
           sign = -1.0;
-          const int node = sm.perm[k][f];
 #pragma unroll
-          for (int c = 0; c < ND; ++c) qL[c] = sm.q[el][node][c];
-          double qg[ND];
-          bc_state<DIM>(mesh, rec, k, NF, qg);
-#pragma unroll
-          for (int c = 0; c < ND; ++c) qR[c] = qg[c];
+          for (int c = 0; c < ND; ++c) {
+            qL[c] = sm.q[el][node][c];
+            qR[c] = qn[c];
+          }
           roe_flux<DIM>(phys, qL, qR, nrm, Fk);
         } else {
-          const int o = (rec.z >> 8) & 0xff, side = (rec.z >> 16) & 1;
-          const int kR = sbp.nbrperm[k * sbp.norient + o];
-          m = side == 0 ? k : kR;
           sign = side == 0 ? -1.0 : 1.0;
-          const int node = sm.perm[m][f];
 #pragma unroll
           for (int c = 0; c < ND; ++c) {
             const T qo = sm.q[el][node][c];
