@@ -170,12 +170,85 @@ def cpu_baseline(budget_s=16.0):
             "host": {"cpu_model": model, "nproc": nproc, "threads_used": threads}}
 
 
+def solves_metrics(seq_sample=12, serve_requests=32, rates=(20.0, 400.0)):
+    """The other two metrics, steady solves/s at ||R|| <= 1e-10 and p50/p99
+    solve latency, with inputs built before each clock starts:
+
+    * batched: the 96 uneven 2D vortex solves (seed 3; n in 8..128, Omega p1-3
+      Roe or diagonal-E p2-3 IR split form, inflow Mach 0.3-0.95, FP32 blocks,
+      ILU(0), GMRES(30)) in one batch; solves/s and the p50/p99 of the times
+      at which instances retire;
+    * one at a time: the plain single-context device path (pdes_newton_solve)
+      on a seeded sample of seq_sample of those instances, timed per solve;
+      its solves/s is the sample's count over its total time;
+    * served: Poisson arrivals of serve_requests mixed 2D/3D requests (seed 4;
+      70% 2D vortex n 32..128, 30% 3D cube n 8..24, Omega p1-2, M 0.3-0.8)
+      into the continuously batching SolveServer at each rate; solves/s and
+      p50/p99 latency from submit to result."""
+    from pdes_amd import euler, linsolve, problems
+    from pdes_amd.batch import BatchSolver
+    import bench_serve
+    import bench_solves
+    opts = bench_solves.newton_options("ilu0", 30)
+    t0 = time.perf_counter()
+    inst = problems.config3_instances()
+    gen = time.perf_counter() - t0
+    print(f"solves: batched 96 instances (built in {gen:.1f} s)", file=sys.stderr, flush=True)
+    bs = BatchSolver([(m, op, ph) for (m, op, ph, q) in inst])
+    t0 = time.perf_counter()
+    for i, (m, op, ph, q0) in enumerate(inst):
+        bs.set_state(i, q0)
+    reps = bs.solve(opts)
+    t_batch = time.perf_counter() - t0
+    bs.free()
+    lat_b = [r.elapsed_s for r in reps]
+    conv_b = sum(1 for r in reps if r.converged == 1 and r.res_norm <= 1e-10)
+    rng = np.random.default_rng(7)
+    sample = sorted(int(i) for i in rng.choice(len(inst), size=seq_sample, replace=False))
+    seq_lat, conv_s = [], 0
+    for i in sample:
+        m, op, ph, q0 = inst[i]
+        e = euler.EulerData(m, op, ph)
+        t1 = time.perf_counter()
+        e.set_q(q0)
+        r = linsolve.newton(m, op, e, opts)
+        seq_lat.append(time.perf_counter() - t1)
+        conv_s += int(r.converged == 1 and r.res_norm <= 1e-10)
+        e.free()
+        print(f"solves: one at a time, instance {i}: {seq_lat[-1]:.2f} s", file=sys.stderr, flush=True)
+    seq_rate = len(sample) / sum(seq_lat)
+    out = {"batched": {"workload": "96 2D vortex instances (seed 3), n 8..128, Omega p1-3 Roe / diagonal-E p2-3 "
+                                   "IR split, inflow Mach 0.3-0.95, Newton to ||R|| <= 1e-10, FP32 blocks, ILU(0), "
+                                   "GMRES(30)",
+                       "solves_per_s": len(inst) / t_batch, "seconds": t_batch, "converged": conv_b,
+                       "retire_p50_s": float(np.percentile(lat_b, 50)),
+                       "retire_p99_s": float(np.percentile(lat_b, 99))},
+           "one_at_a_time": {"sample": sample, "solves_per_s": seq_rate, "seconds": sum(seq_lat),
+                             "converged": conv_s, "latency_p50_s": float(np.percentile(seq_lat, 50)),
+                             "latency_p99_s": float(np.percentile(seq_lat, 99))},
+           "speedup_batched_vs_one_at_a_time": (len(inst) / t_batch) / seq_rate}
+    t0 = time.perf_counter()
+    reqs = bench_serve.make_requests(serve_requests)
+    print(f"solves: served requests built in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    served = []
+    for rate in rates:
+        r = bench_serve.run_rate(reqs, rate, 96 << 30, 4)
+        served.append({k: r[k] for k in ("rate_per_s", "requests", "converged", "solves_per_s", "latency_p50_s",
+                                         "latency_p99_s", "mean_batch_size", "batches", "repacks")})
+        print(f"solves: served at {rate}/s: {served[-1]}", file=sys.stderr, flush=True)
+    out["served"] = {"workload": f"{serve_requests} Poisson-arrival requests (seed 4): 70% 2D vortex n 32..128, "
+                                 "30% 3D cube n 8..24, Omega p1-2 Roe, M 0.3-0.8, aoa +-5 deg, ILU(0), "
+                                 "HBM budget 96 GiB", "rates": served}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=250)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solves", action="store_true", help="skip the steady-solve metrics")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="few copy-kernel launches only (for counter runs)")
     ap.add_argument("--variants", default="all", help="comma list like p2_roe,p4_ir (default all)")
@@ -248,6 +321,7 @@ def main():
                 "traffic": traffic_for(pd, fd)}
     line = {
         "metric": "DOF-residual evals/s",
+        "other_metrics": "steady solves/s at ||R|| <= 1e-10 and p50/p99 solve latency: steady_solves",
         "value": total_dof_evals / (total_ms / 1e3),
         "unit": "DOF-residual evals/s",
         "n_gpus": 1,
@@ -271,6 +345,8 @@ def main():
     cms = euler.copy_bandwidth(copy_bytes, 3 if args.calibrate_copy else 20)
     line["copy_reference"] = {"bytes_read": copy_bytes, "bytes_written": copy_bytes, "kernel_ms": cms,
                               "GBs": 2 * copy_bytes / (cms / 1e3) / 1e9}
+    if not args.no_solves and args.variants == "all" and not args.calibrate_copy:
+        line["steady_solves"] = solves_metrics()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -355,6 +355,46 @@ int pdes_batch_newton_solve(pdes_batch* batch, const pdes_newton_opts* opts,
 /* residual of instance inst at its current state (res_host: its numDof doubles) */
 int pdes_batch_eval_residual(pdes_batch* batch, int inst, double* res_host);
 
+/* Incremental form of pdes_batch_newton_solve, for serving loops (no
+ * counterpart in the reference): runs at most max_passes passes of each
+ * group's continuously batched Newton loop (a pass starts due Newton steps,
+ * begins a GMRES cycle for every linear solve in progress, applies the Newton
+ * updates of the solves that ended and runs the cycle's Arnoldi steps), then
+ * returns.  The first call starts the solve; later calls continue it.
+ * reports[ninst] (may be null) receives every instance's report so far,
+ * active[ninst] (may be null) 1 for instances still being solved, and
+ * *all_done (may be null) 1 once no instance has work left.  Between calls
+ * every instance in a linear solve sits at a GMRES restart. */
+int pdes_batch_advance(pdes_batch* batch, const pdes_newton_opts* opts, int max_passes,
+                       pdes_newton_report* reports, int32_t* active, int* all_done);
+
+/* State of one instance between pdes_batch_advance calls, to move an
+ * unfinished solve into another batch (a serving loop repacks its running
+ * group when requests join or finished ones leave).  With in_solve set the
+ * instance is at a GMRES restart of its current Newton step: its iterate dq,
+ * right-hand side rhs, iteration count and initial residual norm carry over;
+ * the Jacobian and preconditioner are rebuilt from the state, so the solve
+ * continues with the bits it would have had in its old batch. */
+typedef struct {
+  int32_t active;              /* 1: not finished */
+  int32_t failed;
+  int32_t in_solve;            /* in the middle of a linear solve */
+  int32_t newton_steps;        /* Newton steps started */
+  double tau, tau_prev_norm;   /* implicit-Euler pseudo-time step and the norm of its last update */
+  int32_t gmres_iters;         /* GMRES iterations of the current linear solve */
+  int32_t gmres_live;          /* 1 while that solve has not met its tolerance */
+  double gmres_beta0;          /* its initial residual norm */
+  pdes_newton_report report;
+} pdes_batch_inst_state;
+
+/* q, dq, rhs: the instance's numDof doubles (dq and rhs used when in_solve). */
+int pdes_batch_export(pdes_batch* batch, int inst, pdes_batch_inst_state* state, double* q, double* dq,
+                      double* rhs);
+/* Before the batch's first pdes_batch_advance: the instance continues the
+ * exported solve instead of starting a new one. */
+int pdes_batch_import(pdes_batch* batch, int inst, const pdes_batch_inst_state* state, const double* q,
+                      const double* dq, const double* rhs);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,7 @@ def run_rate(reqs, rate, budget, lanes, seed=4):
         "latency_p99_s": float(np.percentile(lat, 99)),
         "mean_batch_size": float(np.mean([t.batch_size for t in tickets])),
         "batches": srv.batches,
+        "repacks": srv.repacks,
         "peak_admitted_bytes": srv.peak_bytes,
     }
 

@@ -301,11 +301,12 @@ struct InstState {
   double tau = 0, tau_prev_norm = 0;  // implicit-Euler pseudo-time step and the norm it was last updated at
   int nit = 0;               // Newton steps taken
   bool in_solve = false;     // its linear system is being solved (GMRES cycles)
-  bool g_iter = false;       // still iterating in the current GMRES cycle
-  bool g_done = false;       // converged in GMRES
-  int g_total = 0, g_j = 0;
-  double g_beta0 = -1, g_res = 0;
-  std::vector<double> H, cs, sn, g;
+  // set by pdes_batch_import: the instance resumes a solve taken over from
+  // another batch (its state, Newton counters and, when in_solve, its GMRES
+  // iterate, right-hand side, iteration count and initial residual)
+  bool imported = false;
+  int32_t g_total = 0, g_live = 1;
+  double g_beta0 = -1;
 };
 
 }  // namespace
@@ -340,6 +341,10 @@ struct pdes_batch_group {
   int32_t* h_flags = nullptr;        // pinned readback of d_iter / d_live / d_total
   std::vector<int32_t> h_active;     // host copy of the ctx's per-instance work-list flags
   int gmres_R = 0;                   // restart length the H/cs/sn/g buffers are sized for
+  // incremental solve (pdes_batch_advance): started after the first pass,
+  // finished when no instance has work left
+  bool started = false, finished = false;
+  std::chrono::steady_clock::time_point t0;
 };
 
 struct pdes_batch {
@@ -492,35 +497,102 @@ int batch_residual_norms(pdes_batch_group* g, std::vector<int>& chunks, std::vec
 // steps for the others.  No instance waits for another's linear solve, and
 // each one's arithmetic is the same as alone: every kernel is masked or
 // listed per instance and every reduction is instance-aligned.
-int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
+int group_finish(pdes_batch_group* g);
+
+double group_elapsed(const pdes_batch_group* g) {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - g->t0).count();
+}
+
+// start of a solve: residual norms of every instance (fresh instances take
+// theirs as res_norm0; imported ones keep their counters), the BSR pattern,
+// and, for imported instances in the middle of a linear solve, their GMRES
+// iterate, right-hand side and counters plus the Jacobian and preconditioner
+// of their current Newton step (both functions of the state only, so the
+// bits are those of the batch they came from)
+int group_init(pdes_batch_group* g, const pdes_newton_opts* o) {
   pdes_ctx* c = g->c;
-  auto t0 = std::chrono::steady_clock::now();
-  auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  g->t0 = std::chrono::steady_clock::now();
+  std::vector<int> chunks, insts;
+  int st;
+  hipStream_t sm = c->stream;
+  PDES_HIP(hipMemsetAsync(c->d_flag, 0, g->inst.size() * 4, sm));
+  for (auto& s : g->inst)
+    if (!s.imported) {
+      s.active = true;
+      s.failed = false;
+      s.rep = pdes_newton_report{};
+      s.tau = 0;
+      s.nit = 0;
+      s.in_solve = false;
+    }
+  for (int pass = 0; pass < 2; ++pass) {
+    const bool fresh = pass == 0;
+    if ((st = set_worklist(g, [&](const InstState& s) { return s.active && s.imported != fresh; }, chunks, insts)))
+      return st;
+    if (insts.empty()) continue;
+    if ((st = batch_residual_norms(g, chunks, insts, fresh))) return st;
+  }
+  for (auto& s : g->inst)
+    if (!s.imported && s.active && s.rep.res_norm < o->res_abstol) {
+      s.active = false;
+      s.rep.converged = 1;
+      s.rep.elapsed_s = group_elapsed(g);
+    }
+  if ((st = jacobian_setup(c, o->fp64_blocks))) return st;
+  if ((st = set_worklist(g, [](const InstState& s) { return s.imported && s.active && s.in_solve; }, chunks, insts)))
+    return st;
+  if (!insts.empty()) {
+    const int ms = g->mstride;
+    std::vector<int32_t> tot(g->inst.size(), 0), live(g->inst.size(), 0);
+    std::vector<double> gs(g->inst.size() * GS_NSCAL, 0.0);
+    for (int i : insts) {
+      tot[i] = g->inst[i].g_total;
+      live[i] = g->inst[i].g_live;
+      gs[(size_t)i * GS_NSCAL + GS_BETA0] = g->inst[i].g_beta0;
+    }
+    PDES_HIP(hipMemcpyAsync(g->d_total, tot.data(), tot.size() * 4, hipMemcpyHostToDevice, sm));
+    PDES_HIP(hipMemcpyAsync(g->d_live, live.data(), live.size() * 4, hipMemcpyHostToDevice, sm));
+    PDES_HIP(hipMemcpyAsync(g->d_gs, gs.data(), gs.size() * 8, hipMemcpyHostToDevice, sm));
+    if (o->globalize_euler) {
+      std::vector<double> taus(g->inst.size(), 1.0);
+      for (size_t i = 0; i < g->inst.size(); ++i)
+        if (g->inst[i].tau > 0) taus[i] = g->inst[i].tau;
+      if ((st = set_ptc_scale(c, taus.data(), (int)taus.size()))) return st;
+    }
+    if ((st = assemble(c))) return st;
+    if ((st = pc_setup(c, o->pc_kind))) return st;
+    (void)ms;
+    PDES_HIP(hipStreamSynchronize(sm));
+  }
+  for (auto& s : g->inst) s.imported = false;
+  g->started = true;
+  g->finished = false;
+  return PDES_OK;
+}
+
+// One pass of the continuously batched Newton loop of a group: each
+// instance advances its own Newton steps.  A pass (1) starts a Newton step
+// (Jacobian, preconditioner, right-hand side) for the instances that need
+// one, (2) begins a GMRES(m) cycle for every instance with a linear system
+// in progress (V_0 = b - A x; an instance that has just started has x = 0,
+// so V_0 = b), (3) takes the Newton update of the instances whose linear
+// solve has ended there (converged or out of iterations) and tests their
+// convergence, and (4) runs the cycle's Arnoldi steps for the others.  No
+// instance waits for another's linear solve, and each one's arithmetic is
+// the same as alone: every kernel is masked or listed per instance and every
+// reduction is instance-aligned.  Between passes every instance in a linear
+// solve sits at a GMRES restart, whose whole state is its iterate, its
+// right-hand side, its iteration count and its initial residual.
+int group_pass(pdes_batch_group* g, const pdes_newton_opts* o) {
+  pdes_ctx* c = g->c;
+  auto elapsed = [&]() { return group_elapsed(g); };
   std::vector<int> chunks, insts;
   int st;
   const int R = o->gmres_restart;
   const int64_t ldv = c->numDof;
   const int ms = g->mstride;
   hipStream_t sm = c->stream;
-  PDES_HIP(hipMemsetAsync(c->d_flag, 0, g->inst.size() * 4, sm));
-  for (auto& s : g->inst) {
-    s.active = true;
-    s.failed = false;
-    s.rep = pdes_newton_report{};
-    s.tau = 0;
-    s.nit = 0;
-    s.in_solve = false;
-  }
-  if ((st = set_worklist(g, [](const InstState& s) { return s.active; }, chunks, insts))) return st;
-  if ((st = batch_residual_norms(g, chunks, insts, true))) return st;
-  for (auto& s : g->inst)
-    if (s.active && s.rep.res_norm < o->res_abstol) {
-      s.active = false;
-      s.rep.converged = 1;
-      s.rep.elapsed_s = elapsed();
-    }
-  if ((st = jacobian_setup(c, o->fp64_blocks))) return st;
-  while (true) {
+  {
     // (1) start a Newton step where one is due
     for (auto& s : g->inst)
       if (s.active && !s.in_solve && s.nit >= o->itermax) {
@@ -566,7 +638,7 @@ int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
     }
     // (2) begin a GMRES cycle for every linear solve in progress
     if ((st = set_worklist(g, [](const InstState& s) { return s.in_solve; }, chunks, insts))) return st;
-    if (insts.empty()) break;
+    if (insts.empty()) return group_finish(g);
     {
       const int64_t n = active_dofs(g);
       if ((st = apply_op_masked(g, o->jv_mode, g->d_dq, g->d_w, g->d_live))) return st;
@@ -623,11 +695,11 @@ int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
           s.rep.elapsed_s = elapsed();
         }
       }
-      if (!any_iter) continue;
+      if (!any_iter) return PDES_OK;
       // back to the solves still iterating in this cycle
       if ((st = set_worklist(g, [](const InstState& s) { return s.in_solve; }, chunks, insts))) return st;
     }
-    if (!any_iter) continue;
+    if (!any_iter) return PDES_OK;
     // (4) the Arnoldi steps of this cycle and the update x += M^-1 V y
     const int64_t n = active_dofs(g);
     const int ni = (int)insts.size(), nch = (int)chunks.size();
@@ -671,12 +743,31 @@ int group_newton(pdes_batch_group* g, const pdes_newton_opts* o) {
     hipLaunchKernelGGL(add_masked, dim3(grid_n(n)), dim3(256), 0, sm, g->d_z, g->d_dq, c->d_el_list, c->d_el_inst,
                        g->dpe, n, g->d_part);
     PDES_HIP(hipGetLastError());
+
   }
-  const double el = elapsed();
+  return PDES_OK;
+}
+
+int group_finish(pdes_batch_group* g) {
+  const double el = group_elapsed(g);
   for (auto& s : g->inst)
     if (s.rep.converged != 1 && s.rep.elapsed_s == 0) s.rep.elapsed_s = el;
-  c->nlist = -1;
-  c->ptc_on = false;
+  g->c->nlist = -1;
+  g->c->ptc_on = false;
+  g->finished = true;
+  g->started = false;
+  return PDES_OK;
+}
+
+// up to max_passes passes of a group (all of them when max_passes <= 0)
+int group_newton(pdes_batch_group* g, const pdes_newton_opts* o, int max_passes) {
+  int st;
+  if (g->finished) return PDES_OK;
+  if (!g->started && (st = group_init(g, o))) return st;
+  for (int k = 0; max_passes <= 0 || k < max_passes; ++k) {
+    if ((st = group_pass(g, o))) return st;
+    if (g->finished) break;
+  }
   return PDES_OK;
 }
 
@@ -938,9 +1029,13 @@ int pdes_batch_eval_residual(pdes_batch* B, int inst, double* res) {
   return PDES_OK;
 }
 
-// Newton for every instance of the batch; the groups run concurrently, each
-// from its own host thread on its own stream.  reports[ninst] (may be null).
-int pdes_batch_newton_solve(pdes_batch* B, const pdes_newton_opts* o, pdes_newton_report* reports) {
+}  // extern "C"
+
+namespace {
+
+// up to max_passes passes of every group (to the end when max_passes <= 0);
+// the groups run concurrently, each from its own host thread on its own stream
+int batch_run(pdes_batch* B, const pdes_newton_opts* o, int max_passes, pdes_newton_report* reports, int* all_done) {
   PDES_CHECK(B && o, PDES_ERR_ARG, "null argument");
   PDES_CHECK(o->gmres_restart > 0 && o->gmres_restart < 63, PDES_ERR_ARG, "gmres_restart must be in 1..62");
   PDES_CHECK(!o->globalize_euler || o->euler_tau > 0, PDES_ERR_ARG, "euler_tau must be positive");
@@ -949,7 +1044,9 @@ int pdes_batch_newton_solve(pdes_batch* B, const pdes_newton_opts* o, pdes_newto
   std::vector<std::thread> th;
   for (size_t gi = 0; gi < B->groups.size(); ++gi) {
     pdes_batch_group* g = B->groups[gi];
-    if (g->restart != o->gmres_restart || !g->d_V) {
+    if (g->started) {
+      PDES_CHECK(g->restart == o->gmres_restart, PDES_ERR_ARG, "gmres_restart changed during a solve");
+    } else if (g->restart != o->gmres_restart || !g->d_V) {
       if (g->d_V) hipFree(g->d_V);
       g->d_V = nullptr;
       g->restart = o->gmres_restart;
@@ -961,7 +1058,7 @@ int pdes_batch_newton_solve(pdes_batch* B, const pdes_newton_opts* o, pdes_newto
       }
     }
     th.emplace_back([&, gi, g]() {
-      status[gi] = group_newton(g, o);
+      status[gi] = group_newton(g, o, max_passes);
       if (status[gi]) errs[gi] = pdes_last_error();
     });
   }
@@ -977,6 +1074,104 @@ int pdes_batch_newton_solve(pdes_batch* B, const pdes_newton_opts* o, pdes_newto
       reports[i] = B->groups[w.first]->inst[w.second].rep;
       if (B->groups[w.first]->inst[w.second].failed) reports[i].converged = -1;
     }
+  if (all_done) {
+    *all_done = 1;
+    for (auto* g : B->groups) *all_done = *all_done && g->finished;
+  }
+  return PDES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Newton for every instance of the batch, from the start.
+int pdes_batch_newton_solve(pdes_batch* B, const pdes_newton_opts* o, pdes_newton_report* reports) {
+  PDES_CHECK(B && o, PDES_ERR_ARG, "null argument");
+  for (auto* g : B->groups) {
+    g->started = false;
+    g->finished = false;
+  }
+  return batch_run(B, o, 0, reports, nullptr);
+}
+
+int pdes_batch_advance(pdes_batch* B, const pdes_newton_opts* o, int max_passes, pdes_newton_report* reports,
+                       int32_t* active, int* all_done) {
+  PDES_CHECK(B && o && max_passes > 0, PDES_ERR_ARG, "bad argument");
+  int st = batch_run(B, o, max_passes, reports, all_done);
+  if (st) return st;
+  if (active)
+    for (int i = 0; i < B->ninst; ++i) {
+      auto w = B->where[i];
+      active[i] = B->groups[w.first]->inst[w.second].active ? 1 : 0;
+    }
+  return PDES_OK;
+}
+
+int pdes_batch_export(pdes_batch* B, int inst, pdes_batch_inst_state* out, double* q, double* dq, double* rhs) {
+  pdes_batch_group* g;
+  InstState* s;
+  int st = locate(B, inst, &g, &s);
+  if (st) return st;
+  PDES_CHECK(out && q, PDES_ERR_ARG, "null argument");
+  pdes_ctx* c = g->c;
+  const int local = B->where[inst].second;
+  const int64_t off = s->el0 * g->dpe, n = (s->el1 - s->el0) * g->dpe;
+  out->active = s->active;
+  out->failed = s->failed;
+  out->in_solve = s->in_solve;
+  out->newton_steps = s->nit;
+  out->tau = s->tau;
+  out->tau_prev_norm = s->tau_prev_norm;
+  out->gmres_iters = 0;
+  out->gmres_live = 1;
+  out->gmres_beta0 = -1.0;
+  out->report = s->rep;
+  PDES_HIP(hipMemcpyAsync(q, c->d_q + off, n * 8, hipMemcpyDeviceToHost, c->stream));
+  if (s->in_solve) {
+    PDES_CHECK(dq && rhs, PDES_ERR_ARG, "an instance in a linear solve needs dq and rhs");
+    double gs[GS_NSCAL];
+    PDES_HIP(hipMemcpyAsync(dq, g->d_dq + off, n * 8, hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipMemcpyAsync(rhs, g->d_rhs + off, n * 8, hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipMemcpyAsync(&out->gmres_iters, g->d_total + local, 4, hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipMemcpyAsync(&out->gmres_live, g->d_live + local, 4, hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipMemcpyAsync(gs, g->d_gs + (size_t)local * GS_NSCAL, sizeof gs, hipMemcpyDeviceToHost, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));
+    out->gmres_beta0 = gs[GS_BETA0];
+  }
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+int pdes_batch_import(pdes_batch* B, int inst, const pdes_batch_inst_state* in, const double* q, const double* dq,
+                      const double* rhs) {
+  pdes_batch_group* g;
+  InstState* s;
+  int st = locate(B, inst, &g, &s);
+  if (st) return st;
+  PDES_CHECK(in && q, PDES_ERR_ARG, "null argument");
+  PDES_CHECK(!g->started, PDES_ERR_ARG, "instances are imported before the batch's first pdes_batch_advance");
+  PDES_CHECK(in->active, PDES_ERR_ARG, "only an instance that has not finished can be imported");
+  pdes_ctx* c = g->c;
+  const int64_t off = s->el0 * g->dpe, n = (s->el1 - s->el0) * g->dpe;
+  s->imported = true;
+  s->active = true;
+  s->failed = false;
+  s->in_solve = in->in_solve != 0;
+  s->nit = in->newton_steps;
+  s->tau = in->tau;
+  s->tau_prev_norm = in->tau_prev_norm;
+  s->g_total = in->gmres_iters;
+  s->g_live = in->gmres_live;
+  s->g_beta0 = in->gmres_beta0;
+  s->rep = in->report;
+  PDES_HIP(hipMemcpyAsync(c->d_q + off, q, n * 8, hipMemcpyHostToDevice, c->stream));
+  if (s->in_solve) {
+    PDES_CHECK(dq && rhs, PDES_ERR_ARG, "an instance in a linear solve needs dq and rhs");
+    PDES_HIP(hipMemcpyAsync(g->d_dq + off, dq, n * 8, hipMemcpyHostToDevice, c->stream));
+    PDES_HIP(hipMemcpyAsync(g->d_rhs + off, rhs, n * 8, hipMemcpyHostToDevice, c->stream));
+  }
+  PDES_HIP(hipStreamSynchronize(c->stream));
   return PDES_OK;
 }
 

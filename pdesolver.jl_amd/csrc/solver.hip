@@ -297,6 +297,10 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
                        const int32_t* __restrict__ inst_active) {
   constexpr int N = NP * ND;
   constexpr int EPB = 256 / N;
+  // entries of this row of D'_e^-1 loaded before the neighbour products, so
+  // their latency overlaps with that dependent chain (all of them up to 72
+  // per row, the first 32 beyond)
+  constexpr int PF = N <= 72 ? N : 32;
   __shared__ float v[EPB][N];
   const int slot = threadIdx.x / N;
   const int r = threadIdx.x - slot * N;
@@ -307,6 +311,12 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
     e = lev_el[s];
     const int inst = el_inst ? el_inst[e] : 0;
     if ((mask && !mask[inst]) || (inst_active && !inst_active[inst])) act = false;
+  }
+  float mrow[PF];
+  if (act) {
+    const float* Mi = inv + e * N * N;
+#pragma unroll
+    for (int cc = 0; cc < PF; ++cc) mrow[cc] = Mi[cc * N + r];
   }
   if (act) {
     const int i = r / ND, a = r - i * ND;
@@ -335,7 +345,9 @@ ilu_sweep_level_kernel(const int32_t* __restrict__ lev_el, int64_t count, const 
   const float* Mi = inv + e * N * N;
   float y = 0.0f;
 #pragma unroll
-  for (int cc = 0; cc < N; ++cc) y += Mi[cc * N + r] * v[slot][cc];
+  for (int cc = 0; cc < PF; ++cc) y += mrow[cc] * v[slot][cc];
+#pragma unroll
+  for (int cc = PF; cc < N; ++cc) y += Mi[cc * N + r] * v[slot][cc];
   if (forward) x[e * N + r] = (double)y;
   else x[e * N + r] -= (double)y;
 }
@@ -371,6 +383,19 @@ ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, con
     const int inst = el_inst ? el_inst[e] : 0;
     if ((mask && !mask[inst]) || (inst_active && !inst_active[inst])) act = false;
   }
+  // this lane's entries of D'_e^-1 (columns l, l + LPR, ...) and the
+  // right-hand side are loaded before the neighbour products, so their
+  // latency overlaps with that dependent chain
+  constexpr int NC = (N + LPR - 1) / LPR;
+  float mrow[NC];
+  double bval = 0.0;
+  if (act) {
+    const float* Mi = inv + e * N * N;
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (l + k * LPR < N) mrow[k] = Mi[(l + k * LPR) * N + r];
+    if (forward && l == 0) bval = b[e * N + r];
+  }
   if (act) {
     const int i = r / ND, a = r - i * ND;
     const int64_t row = e * NP + i;
@@ -393,13 +418,14 @@ ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, con
     double sum = 0.0;
 #pragma unroll
     for (int k = 0; k < LPR; ++k) sum += tp[slot][r][k];
-    v[slot][r] = (float)(forward ? b[e * N + r] - sum : sum);
+    v[slot][r] = (float)(forward ? bval - sum : sum);
   }
   __syncthreads();
   if (act) {
-    const float* Mi = inv + e * N * N;
     float y = 0.0f;
-    for (int cc = l; cc < N; cc += LPR) y += Mi[cc * N + r] * v[slot][cc];
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (l + k * LPR < N) y += mrow[k] * v[slot][l + k * LPR];
     yp[slot][r][l] = y;
   }
   __syncthreads();

@@ -72,6 +72,9 @@ _SIGS = {
     "pdes_batch_get_state": [_vp, C.c_int, _vp],
     "pdes_batch_newton_solve": [_vp, C.POINTER(abi.NewtonOpts), _vp],
     "pdes_batch_eval_residual": [_vp, C.c_int, _vp],
+    "pdes_batch_advance": [_vp, C.POINTER(abi.NewtonOpts), C.c_int, _vp, _vp, C.POINTER(C.c_int)],
+    "pdes_batch_export": [_vp, C.c_int, C.POINTER(abi.BatchInstState), _vp, _vp, _vp],
+    "pdes_batch_import": [_vp, C.c_int, C.POINTER(abi.BatchInstState), _vp, _vp, _vp],
 }
 
 

@@ -67,6 +67,13 @@ class NewtonReport(C.Structure):
                 ("res_norm", C.c_double), ("step_norm", C.c_double), ("elapsed_s", C.c_double)]
 
 
+class BatchInstState(C.Structure):
+    _fields_ = [("active", C.c_int32), ("failed", C.c_int32), ("in_solve", C.c_int32),
+                ("newton_steps", C.c_int32), ("tau", C.c_double), ("tau_prev_norm", C.c_double),
+                ("gmres_iters", C.c_int32), ("gmres_live", C.c_int32), ("gmres_beta0", C.c_double),
+                ("report", NewtonReport)]
+
+
 def dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_dp)

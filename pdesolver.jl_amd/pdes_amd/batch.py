@@ -66,6 +66,43 @@ class BatchSolver:
         check(lib().pdes_batch_newton_solve(self.handle, C.byref(o), C.cast(reps, C.c_void_p)))
         return list(reps)
 
+    def advance(self, opts: NewtonOptions, max_passes: int = 1):
+        """pdes_batch_advance: up to max_passes passes of every group's Newton
+        loop (the first call starts the solve).  Returns (reports, active,
+        done): every instance's NewtonReport so far, a bool per instance that
+        is still being solved, and whether no instance has work left."""
+        o = opts.desc()
+        reps = (abi.NewtonReport * self.n)()
+        active = np.zeros(self.n, dtype=np.int32)
+        done = C.c_int(0)
+        check(lib().pdes_batch_advance(self.handle, C.byref(o), int(max_passes), C.cast(reps, C.c_void_p),
+                                       active.ctypes.data, C.byref(done)))
+        return list(reps), active.astype(bool), bool(done.value)
+
+    def export(self, i):
+        """State of an unfinished instance between advance calls:
+        (BatchInstState, q, dq, rhs); dq and rhs are None unless it is in the
+        middle of a linear solve."""
+        st = abi.BatchInstState()
+        q = np.empty(self.shape(i))
+        dq = np.empty(self.shape(i))
+        rhs = np.empty(self.shape(i))
+        check(lib().pdes_batch_export(self.handle, int(i), C.byref(st), q.ctypes.data, dq.ctypes.data,
+                                      rhs.ctypes.data))
+        if not st.in_solve:
+            dq = rhs = None
+        return st, q, dq, rhs
+
+    def import_(self, i, st, q, dq=None, rhs=None):
+        """Continue an exported solve as instance i (before the first advance)."""
+        qc = np.ascontiguousarray(q, dtype=np.float64)
+        assert qc.shape == self.shape(i)
+        dqc = None if dq is None else np.ascontiguousarray(dq, dtype=np.float64)
+        rc = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
+        check(lib().pdes_batch_import(self.handle, int(i), C.byref(st), qc.ctypes.data,
+                                      None if dqc is None else dqc.ctypes.data,
+                                      None if rc is None else rc.ctypes.data))
+
     def free(self):
         if getattr(self, "handle", None):
             lib().pdes_batch_destroy(self.handle)

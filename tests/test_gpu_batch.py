@@ -98,3 +98,59 @@ def test_batch_failure_isolated():
     for i in (0, 1, 3):
         assert reps[i].converged == 1 and reps[i].res_norm < 1e-10
     bs.free()
+
+
+@pytest.mark.parametrize("pc_kind,globalize", [(2, False), (1, False), (2, True)])
+def test_repack_mid_solve_is_bit_identical(pc_kind, globalize):
+    """pdes_batch_advance / export / import: three same-kind solves advanced a
+    few passes, exported at GMRES restarts, moved (in another order, beside a
+    fresh fourth solve) into a new batch and finished there, give the bits and
+    the Newton / Krylov counts of the same solves run uninterrupted."""
+    opts = linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=30, pc_kind=pc_kind,
+                                  globalize_euler=globalize, euler_tau=1.0)
+    probs = []
+    for i, (n, Min) in enumerate([(10, 0.5), (7, 0.7), (12, 0.4), (6, 0.6)]):
+        m, op, ph = problems.make_problem("vortex", n, 2, "omega", "roe", vortex_Min=Min)
+        probs.append((m, op, ph, problems.vortex_noise_state(m, ph, seed=20 + i)))
+    ref = BatchSolver([(m, op, ph) for (m, op, ph, q) in probs])
+    for i, p in enumerate(probs):
+        ref.set_state(i, p[3])
+    ref_reps = ref.solve(opts)
+    ref_q = [ref.get_state(i) for i in range(len(probs))]
+    ref.free()
+
+    first = BatchSolver([(m, op, ph) for (m, op, ph, q) in probs[:3]])
+    for i in range(3):
+        first.set_state(i, probs[i][3])
+    # one pass at a time until some solve is caught in the middle of a
+    # linear solve (with easy systems a GMRES solve can end within one pass)
+    for _ in range(12):
+        reps, active, done = first.advance(opts, 1)
+        assert not done
+        states = {i: first.export(i) for i in range(3) if active[i]}
+        if any(st[0].in_solve for st in states.values()):
+            break
+    finished = {i: (first.get_state(i), reps[i]) for i in range(3) if not active[i]}
+    moved = sum(1 for i in states if states[i][0].in_solve)
+    print("moved mid-solve:", moved, "finished before the move:", len(finished), flush=True)
+    assert moved >= 1
+    first.free()
+    order = [i for i in (2, 0, 1) if i in states] + [3]
+    second = BatchSolver([probs[i][:3] for i in order])
+    for k, i in enumerate(order):
+        if i == 3:
+            second.set_state(k, probs[3][3])
+        else:
+            second.import_(k, *states[i])
+    while True:
+        reps2, active2, done2 = second.advance(opts, 4)
+        if done2:
+            break
+    for k, i in enumerate(order):
+        finished[i] = (second.get_state(k), reps2[k])
+    second.free()
+    for i in range(4):
+        q, r = finished[i]
+        assert r.converged == 1 and r.res_norm <= 1e-10
+        assert r.iterations == ref_reps[i].iterations and r.total_krylov_iters == ref_reps[i].total_krylov_iters, i
+        assert np.array_equal(q, ref_q[i]), f"instance {i} differs after the move"

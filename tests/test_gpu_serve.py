@@ -67,6 +67,14 @@ def test_footprint_estimate_covers_the_allocations():
     (free memory before creation minus free memory after a solve)."""
     m, op, ph = problems.make_problem("cube", 6, 2, "omega", "roe", Ma=0.5, aoa_deg=2.0)
     est = estimate_bytes(m, op, ph, OPTS)
+    # a small solve of the same kind first: the runtime allocates the
+    # Jacobian kernels' scratch (private segment) at their first launch, once
+    # per process, and that is not part of a solve's footprint
+    mw, opw, phw = problems.make_problem("cube", 1, 2, "omega", "roe", Ma=0.5, aoa_deg=2.0)
+    warm = BatchSolver([(mw, opw, phw)])
+    warm.set_state(0, problems.freestream_bump_state(mw, phw))
+    warm.solve(OPTS)
+    warm.free()
     free0, _ = device_memory()
     bs = BatchSolver([(m, op, ph)])
     bs.set_state(0, problems.freestream_bump_state(m, ph))
@@ -147,4 +155,39 @@ def test_served_full_size_requests_converge_and_equal_solo():
         print(f"solo {i}: iterations {r.iterations} (served {t.report.iterations})", flush=True)
         assert r.iterations == t.report.iterations
         assert np.array_equal(solo.get_state(0), t.q), f"request {i} differs from its solo solve"
+        solo.free()
+
+
+def test_request_joins_a_running_batch():
+    """Continuous batching: a request of the same kind submitted while another
+    is being solved joins that running batch (a repack at a GMRES restart)
+    and is admitted before the first one finishes; both results equal their
+    solo solves bit for bit, and each is handed back as soon as it converges."""
+    import time
+    mA, opA, phA = problems.make_problem("vortex", 64, 2, "omega", "roe", vortex_Min=0.5)
+    qA = problems.vortex_noise_state(mA, phA, seed=31)
+    mB, opB, phB = problems.make_problem("vortex", 8, 2, "omega", "roe", vortex_Min=0.7)
+    qB = problems.vortex_noise_state(mB, phB, seed=32)
+    srv = SolveServer(1 << 36, lanes=2, opts=OPTS, passes=1)
+    tA = srv.submit(mA, opA, phA, qA)
+    t_wait = time.perf_counter()
+    while tA.t_admit == 0.0 and time.perf_counter() - t_wait < 60:
+        time.sleep(0.01)
+    time.sleep(0.2)
+    assert not tA.done(), "the first request finished before the second could join"
+    tB = srv.submit(mB, opB, phB, qB)
+    assert tA.wait(120.0) and tB.wait(120.0)
+    srv.shutdown()
+    print("A admitted", tA.t_admit - tA.t_submit, "done", tA.t_done - tA.t_submit, "B admitted",
+          tB.t_admit - tA.t_submit, "done", tB.t_done - tA.t_submit, "repacks", srv.repacks, flush=True)
+    assert tB.t_admit < tA.t_done
+    assert srv.repacks >= 1
+    assert tB.t_done < tA.t_done          # the small request leaves as soon as it converges
+    for t, (m, op, ph, q0) in ((tA, (mA, opA, phA, qA)), (tB, (mB, opB, phB, qB))):
+        assert t.error is None and t.report.converged == 1
+        solo = BatchSolver([(m, op, ph)])
+        solo.set_state(0, q0)
+        r = solo.solve(OPTS)[0]
+        assert r.iterations == t.report.iterations
+        assert np.array_equal(solo.get_state(0), t.q)
         solo.free()
