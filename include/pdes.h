@@ -141,9 +141,12 @@ int pdes_state_ptr(pdes_ctx* ctx, double** q_dev, double** res_dev);
 int pdes_eval_residual(pdes_ctx* ctx, double t);
 
 /* Launch tuning (no counterpart in the reference): the residual kernel has
- * NUM variants (elements per workgroup x waves per SIMD); pdes_autotune times
- * each on this context's mesh, for the plain and the RK4-stage form, and keeps
- * the fastest (chosen[0], chosen[1]).  pdes_set_launch_variant fixes them. */
+ * launch variants 0-3 (elements per workgroup 256/Np or 128/Np x 3 or 4 waves
+ * per SIMD) and, for the weak form, 4-7: the same shapes with the volume term
+ * Q^T F on the FP64 matrix cores (the split form runs variant v & 3 for
+ * these); pdes_autotune times each on this context's mesh, for the plain and
+ * the RK4-stage form, and keeps the fastest (chosen[0], chosen[1]).
+ * pdes_set_launch_variant (0..7) fixes them. */
 int pdes_autotune(pdes_ctx* ctx, int iters, int32_t* chosen);
 int pdes_set_launch_variant(pdes_ctx* ctx, int plain_variant, int rk_variant);
 

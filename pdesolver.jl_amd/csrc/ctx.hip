@@ -97,9 +97,14 @@ int pdes_ctx_create(const pdes_mesh_desc* m, const pdes_sbp_desc* s, const pdes_
   c->pc.rho_free = ph->rho_free;
   c->pc.E_free = ph->p_free / (ph->gamma - 1) + 0.5 * ph->rho_free * ph->Ma * ph->Ma;  // types.jl:252
   c->pc.vortex_Min = ph->vortex_Min;
-  // default launch variants measured on MI355X: the weak form runs best at 3
-  // waves per SIMD, the register-heavier split form at 2
-  c->res_variant[0] = c->res_variant[1] = (ph->volume_integral_type == 2) ? 0 : 1;
+  // default launch variants measured on MI355X: the split form at 3 waves per
+  // SIMD; the weak form at 4, with Q^T F on the matrix cores where its product
+  // lays over the flux planes in LDS ((dim - 1) Np >= nfaces Nf), else on the
+  // vector ALUs (the product's own LDS array costs a block per CU)
+  if (ph->volume_integral_type == 2)
+    c->res_variant[0] = c->res_variant[1] = 0;
+  else
+    c->res_variant[0] = c->res_variant[1] = ((c->dim - 1) * c->Np >= c->nfaces * c->Nf) ? 5 : 1;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     delete c;

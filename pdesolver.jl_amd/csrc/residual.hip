@@ -125,7 +125,7 @@ int pdes_autotune(pdes_ctx* c, int iters, int32_t* chosen) {
   for (int rkmode = 0; rkmode < 2; ++rkmode) {
     float best = 1e30f;
     int bestv = c->res_variant[rkmode];
-    for (int var = 0; var < NUM_RES_VARIANTS; ++var) {
+    for (int var = 0; var < res_variant_count(c); ++var) {
       c->res_variant[rkmode] = var;
       // stage-1 epilogue: reads q and the accumulator, writes scratch buffers only
       RKStage rk{c->d_Minv_node, c->d_q, c->d_work, c->d_work2, 2.0, 0.0, 0.0, 1};

@@ -158,7 +158,7 @@ struct RKStage {
   int32_t stage;                 // 0..3
 };
 
-template <class Op, int VOL, int FLUX, int EPB, class T>
+template <class Op, int VOL, int FLUX, int EPB, class T, bool MF = false>
 struct ResidualSmem {
   static constexpr int NP = Op::NP, ND = Op::ND, DIM = Op::DIM, NF = Op::NF, NFACES = Op::NFACES;
   // weak form: Q_d[j][i]; split form: 2 S by circulant distance, [d][dd][i]
@@ -192,17 +192,97 @@ struct ResidualSmem {
   static constexpr int G_C_BUF = ((NP / 2) + 1) & 1;
   static_assert(!G_IN_F || DIM * NP >= NFACES * NF, "face contributions must fit the weak-form flux planes");
   T G[(G_IN_F || G_IN_C) ? 1 : EPB][NFACES][NF][ND];
+  // weak form on the matrix cores: the volume term of every (element, node)
+  // as planes [component][element][node], laid over the flux planes above
+  // the face contributions where they fit there (the flux planes are dead
+  // once the product is formed), else in an array of their own
+  static constexpr bool RV_IN_F = (DIM - 1) * NP >= NFACES * NF;
+  static constexpr int RV_OFF = ND * EPB * NFACES * NF;
+  double Rv[(MF && !RV_IN_F) ? ND : 1][(MF && !RV_IN_F) ? EPB * NP : 1];
 };
 
-template <class Op, int VOL, int FLUX, int EPB, class T, bool RK = false, int WAVES = 2>
+// Weak-form volume term res_i = sum_d sum_j Q_d[j][i] F_d[j] of all EPB
+// elements of a block as one product on the FP64 matrix cores,
+//   R (NP x ND*EPB) = A (NP x DIM*NP) . B (DIM*NP x ND*EPB),
+//   A[i][(d, j)] = Q_d[j][i] (shared by every element of the block),
+//   B[(d, j)][(k, el)] = component k of element el's flux in direction d at node j,
+// in 16 x 16 output tiles with V_MFMA_F64_16X16X4_F64, k-steps of 4 in the
+// order (d, j) of the vector loop (the instruction accumulates its four
+// products in order, each fused, so R has the vector loop's bits).  Lane l of
+// a wave supplies A[l % 16][l / 16] and B[l / 16][l % 16] of each step and
+// holds D[4 r + l / 16][l % 16], r < 4, of the tile.  Every thread of the
+// block calls compute() (whole waves: the matrix instruction reads all 64
+// lanes); tile t goes to wave t % NW.  store() writes the tiles as planes
+// [component][element][node]; the caller puts a barrier between the two when
+// the planes overlay the flux planes that compute() reads.
+template <int DIM, int NP, int ND, int EPB>
+struct WeakVolumeMfma {
+  typedef double v4d __attribute__((ext_vector_type(4)));
+  static constexpr int PL = EPB * NP;
+  static constexpr int KK = DIM * NP, KS = (KK + 3) / 4;
+  static constexpr int NCOL = ND * EPB, MT = (NP + 15) / 16, NT = (NCOL + 15) / 16;
+  static constexpr int NW = (EPB * NP + 63) / 64;           // waves per block
+  static constexpr int TPW = (MT * NT + NW - 1) / NW;       // tiles per wave (at most)
+  v4d acc[TPW];
+
+  __device__ __forceinline__ void compute(const double* __restrict__ Q, const double* __restrict__ fpl, int tid) {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, q4 = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      acc[u] = v4d{0.0, 0.0, 0.0, 0.0};
+      const int t = wave + u * NW;
+      if (t >= MT * NT) continue;                 // wave-uniform
+      const int mt = t / NT, nt = t - mt * NT;
+      const int row = mt * 16 + l16;             // A row this lane supplies
+      const int col = nt * 16 + l16;             // B column this lane supplies
+      const int kc = col / EPB, ec = col - kc * EPB;
+      const bool rok = row < NP, cok = col < NCOL;
+#pragma unroll 2
+      for (int ks = 0; ks < KS; ++ks) {
+        const int kk = ks * 4 + q4;
+        const int d = kk / NP, j = kk - d * NP;
+        double a = 0.0, b = 0.0;
+        if (kk < KK) {
+          if (rok) a = Q[(d * NP + j) * NP + row];
+          if (cok) b = fpl[(d * ND + kc) * PL + ec * NP + j];
+        }
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(double* __restrict__ rpl, int tid) const {
+    const int lane = tid & 63, wave = tid >> 6;
+    const int l16 = lane & 15, q4 = lane >> 4;
+#pragma unroll
+    for (int u = 0; u < TPW; ++u) {
+      const int t = wave + u * NW;
+      if (t >= MT * NT) continue;
+      const int mt = t / NT, nt = t - mt * NT;
+      const int col = nt * 16 + l16;
+      if (col >= NCOL) continue;
+      const int kc = col / EPB, ec = col - kc * EPB;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int node = mt * 16 + 4 * r + q4;
+        if (node < NP) rpl[kc * PL + ec * NP + node] = acc[u][r];
+      }
+    }
+  }
+};
+
+template <class Op, int VOL, int FLUX, int EPB, class T, bool RK = false, int WAVES = 2, bool MF = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
                       const double* __restrict__ qin, const double* __restrict__ vin,
                       double* __restrict__ res, double* __restrict__ jv, int32_t* __restrict__ flag,
                       RKStage rk) {
   constexpr int DIM = Op::DIM, NP = Op::NP, NF = Op::NF, ST = Op::ST, ND = Op::ND, NFACES = Op::NFACES;
-  using Smem = ResidualSmem<Op, VOL, FLUX, EPB, T>;
+  using Smem = ResidualSmem<Op, VOL, FLUX, EPB, T, MF>;
   __shared__ Smem sm;
+  static_assert(!MF || (VOL == VOL_WEAK && std::is_same<T, double>::value),
+                "the matrix-core volume term is the FP64 weak form");
 
   const int tid = threadIdx.x;
   const int el = tid / NP;
@@ -271,6 +351,11 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   T ri[ND];
 #pragma unroll
   for (int k = 0; k < ND; ++k) ri[k] = 0.0;
+  // matrix-core weak form: the product's tiles, and where they are stored
+  // (read back into ri after the face phase's barrier)
+  WeakVolumeMfma<DIM, NP, ND, EPB> mfma;
+  double* rvpl = nullptr;
+  if constexpr (MF) rvpl = Smem::RV_IN_F ? &sm.F[0][0] + Smem::RV_OFF : &sm.Rv[0][0];
 
   if constexpr (VOL == VOL_WEAK) {
     // getEulerFlux: flux in parametric direction d uses nrm = dxidx[d, :, j, e]
@@ -288,7 +373,9 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
     }
     __syncthreads();
     // weakdifferentiate!(trans=true): res_i += sum_d sum_j Q_d[j][i] F_d[j]
-    if (active) {
+    if constexpr (MF) {
+      mfma.compute(sm.Q, zpl, tid);
+    } else if (active) {
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
 #pragma unroll 4
@@ -299,6 +386,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
         }
     }
     __syncthreads();   // the face phase writes its contributions into the flux planes
+    if constexpr (MF) mfma.store(rvpl, tid);
   } else {
     // split form (calcVolumeIntegralsSplitFormLinear): per-node IR z-variables in LDS,
     // each unordered node pair evaluated once by the node that starts its circulant distance.
@@ -686,6 +774,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   }
   __syncthreads();
   if (active) {
+    if constexpr (MF) {
+#pragma unroll
+      for (int k = 0; k < ND; ++k) ri[k] = rvpl[k * PL + me + i];
+    }
     // res_i += sum_f sum_m interp[s(i,f)][m] G[f][m]
 #pragma unroll
     for (int f = 0; f < NFACES; ++f) {

@@ -6,21 +6,23 @@
 
 namespace pdes {
 
-template <class Op, int VOL, int FLUX, class T, int EPB, int WAVES>
+template <class Op, int VOL, int FLUX, class T, int EPB, int WAVES, bool MF = false>
 static int launch_cfg(pdes_ctx* c, const double* q, const double* v, double* res, double* jv, int32_t* flag,
                       const RKStage* rk) {
-  const int threads = EPB * Op::NP;
+  // the matrix-core variants run whole waves (the extra lanes only stage
+  // tables and join the matrix products)
+  const int threads = MF ? (int)ceil_div(EPB * Op::NP, 64) * 64 : EPB * Op::NP;
   if (c->active_count() == 0) return PDES_OK;
   const int blocks = (int)ceil_div(c->active_count(), EPB);
   if constexpr (std::is_same<T, double>::value) {
     if (rk) {
-      hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T, true, WAVES>), dim3(blocks), dim3(threads), 0,
+      hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T, true, WAVES, MF>), dim3(blocks), dim3(threads), 0,
                          c->stream, c->devmesh(), c->devsbp(), c->pc, q, v, res, jv, flag, *rk);
       PDES_HIP(hipGetLastError());
       return PDES_OK;
     }
   }
-  hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T, false, WAVES>), dim3(blocks), dim3(threads), 0,
+  hipLaunchKernelGGL((euler_residual_kernel<Op, VOL, FLUX, EPB, T, false, WAVES, MF>), dim3(blocks), dim3(threads), 0,
                      c->stream, c->devmesh(), c->devsbp(), c->pc, q, v, res, jv, flag, RKStage{});
   PDES_HIP(hipGetLastError());
   return PDES_OK;
@@ -36,7 +38,16 @@ static int launch_one(pdes_ctx* c, const double* q, const double* v, double* res
   if constexpr (!std::is_same<T, double>::value) {
     return launch_cfg<Op, VOL, FLUX, T, HALF, 2>(c, q, v, res, jv, flag, nullptr);
   } else {
-    switch (variant) {
+    if constexpr (VOL == VOL_WEAK) {
+      switch (variant) {
+        case 4: return launch_cfg<Op, VOL, FLUX, T, FULL, 3, true>(c, q, v, res, jv, flag, rk);
+        case 5: return launch_cfg<Op, VOL, FLUX, T, FULL, 4, true>(c, q, v, res, jv, flag, rk);
+        case 6: return launch_cfg<Op, VOL, FLUX, T, HALF, 3, true>(c, q, v, res, jv, flag, rk);
+        case 7: return launch_cfg<Op, VOL, FLUX, T, HALF, 4, true>(c, q, v, res, jv, flag, rk);
+        default: break;
+      }
+    }
+    switch (variant & 3) {
       case 0: return launch_cfg<Op, VOL, FLUX, T, FULL, 3>(c, q, v, res, jv, flag, rk);
       case 1: return launch_cfg<Op, VOL, FLUX, T, FULL, 4>(c, q, v, res, jv, flag, rk);
       case 2: return launch_cfg<Op, VOL, FLUX, T, HALF, 3>(c, q, v, res, jv, flag, rk);

@@ -140,7 +140,7 @@ def test_rk4_matches_oracle(oracle, p, flux, n):
         k4 = f(q + h * k3)
         q = q + h / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
     first = None
-    for var in range(4):
+    for var in range(8 if flux == "roe" else 4):
         print(f"case p={p} {flux} n={n} rk variant {var}", flush=True)
         check(lib().pdes_set_launch_variant(eqn.ctx, 0, var))
         eqn.set_q(q0)
@@ -198,18 +198,31 @@ def test_per_node_metric_parity(oracle, kind, n, p, opk, flux):
     eqn.free()
 
 
-def test_every_launch_variant_parity(oracle):
-    """The autotuner's launch variants (elements per workgroup x waves per
-    SIMD) all give the oracle's residual; autotune picks a valid one."""
+VARIANT_CASES = [("vortex", 6, 2, "diage", "ir"), ("vortex", 6, 2, "diage", "roe"), ("vortex", 4, 4, "diage", "roe"),
+                 ("vortex", 5, 3, "diage", "roe"), ("vortex", 5, 1, "diage", "roe"), ("vortex", 5, 4, "omega", "roe"),
+                 ("vortex", 5, 2, "omega", "roe"), ("cube", 2, 2, "omega", "roe"), ("cube", 2, 1, "omega", "roe")]
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", VARIANT_CASES)
+def test_every_launch_variant_parity(oracle, kind, n, p, opk, flux):
+    """The autotuner's launch variants all give the oracle's residual, with
+    the same bits: 0-3 (elements per workgroup x waves per SIMD) and, for the
+    weak form, 4-7, the same shapes with Q^T F on the FP64 matrix cores (the
+    matrix instruction accumulates its products in the vector loop's order,
+    each fused, so the result is the vector loop's; Omega p=2/tet shapes keep
+    the product in an LDS array of its own, the others lay it over the flux
+    planes).  The split form runs variant v & 3 for v >= 4.  autotune picks
+    a valid variant."""
     from pdes_amd._lib import check, lib
-    m, op, ph, b = setup("vortex", 6, 2, "diage", "ir")
-    q = state_for("vortex", m, ph)
+    print(f"case {kind} n={n} p={p} {opk} {flux}", flush=True)
+    m, op, ph, b = setup(kind, n, p, opk, flux)
+    q = state_for(kind, m, ph)
     ref = oracle.residual(b, q)
     scale = oracle.residual_scale(b, q)
     eqn = euler.EulerData(m, op, ph)
     eqn.set_q(q)
     first = None
-    for var in range(4):
+    for var in range(8):
         check(lib().pdes_set_launch_variant(eqn.ctx, var, var))
         euler.evalResidual(m, op, eqn)
         assert_parity(eqn.res, ref, scale, 1e-12, f"variant {var}")
@@ -219,5 +232,6 @@ def test_every_launch_variant_parity(oracle):
         else:
             assert np.array_equal(eqn.res, first), f"variant {var} differs from variant 0"
     chosen = euler.autotune(eqn, 2)
-    assert all(0 <= v < 4 for v in chosen)
+    assert all(0 <= v < (8 if flux == "roe" else 4) for v in chosen)
+    assert lib().pdes_set_launch_variant(eqn.ctx, 8, 0) != 0
     eqn.free()
