@@ -21,6 +21,10 @@
 namespace pdes {
 int pc_setup(pdes_ctx* c, int kind);
 int pc_apply(pdes_ctx* c, const double* b, double* x);
+int pc_check(pdes_ctx* c);
+int ilu_range_levels(const pdes_ctx* c, int64_t el0, int64_t el1, int* lf, int* lb);
+int ilu_hint(pdes_ctx* c, const int32_t* inst_sel, int ninst, int64_t nel, int lf, int lb);
+void ilu_hint_clear(pdes_ctx* c);
 }  // namespace pdes
 
 namespace {
@@ -307,6 +311,8 @@ struct InstState {
   bool imported = false;
   int32_t g_total = 0, g_live = 1;
   double g_beta0 = -1;
+  // ILU(0) dependency levels of the instance's elements (forward, backward; -1: not yet known)
+  int ilu_lf = -1, ilu_lb = -1;
 };
 
 }  // namespace
@@ -656,6 +662,7 @@ int group_pass(pdes_batch_group* g, const pdes_newton_opts* o) {
                          c->d_el_list, c->d_el_inst, g->dpe, n, g->d_iter);
       PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_iter, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
       PDES_HIP(hipStreamSynchronize(sm));
+      if ((st = pc_check(c))) return st;
     }
     // (3) Newton update of the instances whose linear solve has ended
     std::vector<char> ended(g->inst.size(), 0);
@@ -704,6 +711,25 @@ int group_pass(pdes_batch_group* g, const pdes_newton_opts* o) {
     const int64_t n = active_dofs(g);
     const int ni = (int)insts.size(), nch = (int)chunks.size();
     const unsigned gi = (unsigned)ceil_div(ni, 64);
+    // the preconditioner's sweeps touch only these instances' elements
+    struct HintGuard {
+      pdes_ctx* c;
+      ~HintGuard() { ilu_hint_clear(c); }
+    } hint_guard{c};
+    if (o->pc_kind == PDES_PC_ILU0) {
+      int64_t hel = 0;
+      int hlf = 0, hlb = 0;
+      std::vector<int32_t> sel(g->inst.size(), 0);
+      for (int i : insts) {
+        InstState& s = g->inst[i];
+        if (s.ilu_lf < 0 && ilu_range_levels(c, s.el0, s.el1, &s.ilu_lf, &s.ilu_lb)) s.ilu_lf = s.ilu_lb = -1;
+        hel += s.el1 - s.el0;
+        hlf = std::max(hlf, s.ilu_lf);
+        hlb = std::max(hlb, s.ilu_lb);
+        sel[i] = 1;
+      }
+      if (hlf > 0 && (st = ilu_hint(c, sel.data(), (int)sel.size(), hel, hlf, hlb))) return st;
+    }
     for (int j = 0; j < R; ++j) {
       double* Vj = g->d_V + (int64_t)j * ldv;
       double* Vn = g->d_V + (int64_t)(j + 1) * ldv;

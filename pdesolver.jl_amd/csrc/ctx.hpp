@@ -72,6 +72,13 @@ struct pdes_ctx {
   // replay multi-launch sequences (ILU sweeps) from captured HIP graphs;
   // off when the environment variable PDES_NO_GRAPHS is set at creation
   bool use_graphs = true;
+  // the elements an ILU(0) application will touch and the dependency levels
+  // they span (set by the batched solver around its Arnoldi steps; -1: all);
+  // ilu_sel: the dependency-driven sweeps run over the level-ordered lists
+  // compacted to those elements
+  int64_t ilu_hint_el = -1;
+  int ilu_hint_lf = 0, ilu_hint_lb = 0;
+  bool ilu_sel = false;
   int partial_n = 0;
   // residual launch variants (see residual_launch.hpp) for the plain and the
   // RK-epilogue kernel, chosen by the autotuner or the default table

@@ -13,8 +13,11 @@
 // Newton follows newtonInner (src/NonlinearSolvers/newton.jl:137-304).
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "jacobian.hpp"
 
@@ -30,6 +33,31 @@ struct pdes_precond {
   int32_t* d_fel = nullptr;
   int32_t* d_bel = nullptr;
   int32_t* d_own = nullptr;   // per node row: offset of the element's own block run within the row
+  // dependency-driven sweeps: per-element completion flags (the epoch of the
+  // sweep that last finished the element) and the ticket / epoch / error words
+  int32_t* d_done = nullptr;
+  int32_t* d_sync = nullptr;
+  // sweep choice: 0 by the cost model, 1 always level by level
+  // (PDES_ILU_LEVELS set), 2 always dependency-driven (PDES_ILU_DEP set)
+  int force = 0;
+  std::vector<int32_t> h_fl, h_bl;   // forward / backward level of each element
+  // the dependency-driven sweeps' level-ordered lists, every level padded
+  // with -1 to whole chunks of `chunk` elements, and their chunk counts
+  int chunk = 1;
+  int32_t* d_fpad = nullptr;
+  int32_t* d_bpad = nullptr;
+  int64_t fpad_n = 0, bpad_n = 0;
+  // the chunks holding elements of selected instances (packed batches), their
+  // counts, the selection flags and the select's work space
+  int32_t* d_fsel = nullptr;
+  int32_t* d_bsel = nullptr;
+  int64_t sel_nf = 0, sel_nb = 0;
+  int32_t* d_inst_sel = nullptr;
+  int32_t* d_nsel = nullptr;
+  void* d_select_tmp = nullptr;
+  size_t select_tmp_bytes = 0;
+  int sel_ninst = 0;
+  double blk_per_el = 0;             // neighbour blocks of an element on one side (mean)
   // the 2 x levels sweep launches of one apply, captured once per (mask,
   // matrix) into a HIP graph over fixed staging vectors and replayed
   double* d_stage_in = nullptr;
@@ -56,6 +84,15 @@ int precond_free(pdes_ctx* c) {
   if (c->pc_obj->d_fel) hipFree(c->pc_obj->d_fel);
   if (c->pc_obj->d_bel) hipFree(c->pc_obj->d_bel);
   if (c->pc_obj->d_own) hipFree(c->pc_obj->d_own);
+  if (c->pc_obj->d_done) hipFree(c->pc_obj->d_done);
+  if (c->pc_obj->d_fpad) hipFree(c->pc_obj->d_fpad);
+  if (c->pc_obj->d_bpad) hipFree(c->pc_obj->d_bpad);
+  if (c->pc_obj->d_fsel) hipFree(c->pc_obj->d_fsel);
+  if (c->pc_obj->d_bsel) hipFree(c->pc_obj->d_bsel);
+  if (c->pc_obj->d_inst_sel) hipFree(c->pc_obj->d_inst_sel);
+  if (c->pc_obj->d_nsel) hipFree(c->pc_obj->d_nsel);
+  if (c->pc_obj->d_select_tmp) hipFree(c->pc_obj->d_select_tmp);
+  if (c->pc_obj->d_sync) hipFree(c->pc_obj->d_sync);
   if (c->pc_obj->d_stage_in) hipFree(c->pc_obj->d_stage_in);
   if (c->pc_obj->d_stage_out) hipFree(c->pc_obj->d_stage_out);
   for (auto& g : c->pc_obj->graphs) hipGraphExecDestroy(g.exec);
@@ -438,6 +475,191 @@ ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, con
   }
 }
 
+// ---------------------------------------------------------------------------
+// Dependency-driven sweeps: one launch per triangle instead of one per level.
+// The level-ordered element list is cut into chunks of ILU_CHUNK elements of
+// one level each (every level padded to whole chunks with -1), so the
+// elements of a chunk never depend on each other.  Each workgroup takes one
+// chunk, in ticket order: an atomic counter hands out chunk positions, so
+// every element that a chunk's elements depend on (their lower neighbours in
+// the forward sweep, their upper ones in the backward sweep) sits in a chunk
+// with a smaller ticket, which already runs on the device or is done.  The
+// first thread of each element waits for those neighbours' completion flags
+// to carry this sweep's epoch, the block computes x_e as the level kernels do
+// (same lanes, same order of operations: the same bits), stores it, waits for
+// its stores to complete and then sets e's flag.  The neighbours' x are read
+// and the own x written with device-scope atomic loads/stores, so no stale
+// copy in another XCD's L2 is ever read.  The flags are never cleared: the
+// epoch, advanced by ilu_sweep_begin before each sweep, tells this sweep's
+// flags from older ones.  A wait gives up after a bounded number of polls,
+// sets the error word and carries on, so every launch ends.
+// ---------------------------------------------------------------------------
+constexpr int ILU_WAIT_POLLS = 1 << 18;
+enum { ILU_SYNC_TICKET = 0, ILU_SYNC_EPOCH = 1, ILU_SYNC_ERR = 2, ILU_SYNC_WORDS = 4 };
+
+// elements per chunk for a row block of n unknowns: as many element row
+// blocks as fill a workgroup of at most 256 threads, at most 4
+__host__ __device__ constexpr int ilu_dep_lanes(int n, bool diage) {
+  return diage ? 1 : (256 / n >= 8 ? 8 : 256 / n);
+}
+__host__ __device__ constexpr int ilu_dep_chunk(int n, bool diage) {
+  return 256 / (n * ilu_dep_lanes(n, diage)) >= 4 ? 4 : 256 / (n * ilu_dep_lanes(n, diage));
+}
+
+__global__ void ilu_sweep_begin(int32_t* __restrict__ sync) {
+  sync[ILU_SYNC_TICKET] = 0;
+  sync[ILU_SYNC_EPOCH] = sync[ILU_SYNC_EPOCH] + 1;
+}
+
+__device__ __forceinline__ double x_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void x_store(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NP, int ND, int NS, bool DIAGE, class BT, int LPR, int CH>
+__global__ void __launch_bounds__(256)
+ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restrict__ chunks, int64_t nchunks,
+                     const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
+                     const int32_t* __restrict__ own, const BT* __restrict__ vals, const float* __restrict__ inv,
+                     const double* __restrict__ b, double* __restrict__ x, int forward,
+                     const int32_t* __restrict__ el_inst, const int32_t* __restrict__ mask,
+                     const int32_t* __restrict__ inst_active, const pdes::ElFace* __restrict__ elface, int nfaces,
+                     int32_t* __restrict__ done, int32_t* __restrict__ sync) {
+  constexpr int N = NP * ND;
+  constexpr int TPE = N * LPR;
+  static_assert(CH * TPE <= 256, "a chunk's rows must fit a workgroup");
+  // neighbour blocks of one row on one side: at most one per face with
+  // diagonal-E faces, the face stencil per face otherwise; this lane's share
+  // is loaded while the waits run
+  constexpr int NFACES = (ND == 4) ? 3 : 4;
+  constexpr int MAXB = DIAGE ? NFACES : (NFACES * NS + LPR - 1) / LPR;
+  __shared__ double tp[CH][N][LPR];
+  __shared__ float yp[CH][N][LPR];
+  __shared__ float v[CH][N];
+  __shared__ int64_t sh_e[CH];
+  __shared__ int sh_any;
+  const int t = threadIdx.x;
+  const int slot = t / TPE;
+  const int tt = t - slot * TPE;
+  const int r = tt / LPR, l = tt - r * LPR;
+  const int32_t epoch = sync[ILU_SYNC_EPOCH];
+  if (t == 0) {
+    const int64_t tk = __hip_atomic_fetch_add(&sync[ILU_SYNC_TICKET], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int64_t ch = tk < nchunks ? (chunks ? (int64_t)chunks[tk] : tk) : -1;
+    int any = 0;
+    for (int sl = 0; sl < CH; ++sl) {
+      int64_t e = ch >= 0 ? (int64_t)order[ch * CH + sl] : -1;
+      if (e >= 0) {
+        const int inst = el_inst ? el_inst[e] : 0;
+        if ((mask && !mask[inst]) || (inst_active && !inst_active[inst])) e = -1;   // no element waits on it
+      }
+      sh_e[sl] = e;
+      any |= e >= 0;
+    }
+    sh_any = any;
+  }
+  __syncthreads();
+  if (!sh_any) return;   // uniform over the block
+  const int64_t e = slot < CH ? sh_e[slot] : -1;
+  const bool act = e >= 0;
+  // everything that does not depend on the neighbours' x: this lane's
+  // entries of D'_e^-1 (columns l, l + LPR, ...; the first 32 when there are
+  // more than 72), the right-hand side (forward) or the own x (backward),
+  // and this lane's neighbour blocks and their columns
+  constexpr int NC = (N + LPR - 1) / LPR;
+  constexpr int PF = NC <= 72 ? NC : 32;
+  float mrow[PF];
+  BT pre[MAXB][ND];
+  int32_t pcol[MAXB];
+  double bval = 0.0;
+  const float* Mi = inv + (act ? e : 0) * N * N;
+  int64_t q0 = 0, q1 = 0;
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+      if (l + k * LPR < N) mrow[k] = Mi[(l + k * LPR) * N + r];
+    if (l == 0) bval = forward ? b[e * N + r] : x[e * N + r];
+    const int i = r / ND, a = r - i * ND;
+    const int64_t row = e * NP + i;
+    const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
+    q0 = forward ? p0 : p0 + own[row] + NP;
+    q1 = forward ? p0 + own[row] : p1;
+#pragma unroll
+    for (int u = 0; u < MAXB; ++u) {
+      const int64_t p = q0 + l + (int64_t)u * LPR;
+      pcol[u] = p < q1 ? colind[p] : 0;
+#pragma unroll
+      for (int c = 0; c < ND; ++c) pre[u][c] = p < q1 ? vals[p * ND * ND + a * ND + c] : BT(0);
+    }
+  }
+  if (act && tt == 0) {
+    for (int f = 0; f < nfaces; ++f) {
+      const int64_t k = elface[e * nfaces + f].x;
+      if (!(forward ? (k >= 0 && k < e) : (k > e))) continue;
+      int polls = 0;
+      while (__hip_atomic_load(&done[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (++polls > ILU_WAIT_POLLS) {
+          __hip_atomic_store(&sync[ILU_SYNC_ERR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  __syncthreads();
+  if (act) {
+    const int a = r % ND;
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < MAXB; ++u) {
+      if (q0 + l + (int64_t)u * LPR < q1) {
+        const double* xc = x + (int64_t)pcol[u] * ND;
+#pragma unroll
+        for (int c = 0; c < ND; ++c) acc += (double)pre[u][c] * x_load(xc + c);
+      }
+    }
+    for (int64_t p = q0 + l + (int64_t)MAXB * LPR; p < q1; p += LPR) {
+      const int32_t col = colind[p];
+      const BT* blk = vals + p * ND * ND + a * ND;
+      const double* xc = x + (int64_t)col * ND;
+#pragma unroll
+      for (int c = 0; c < ND; ++c) acc += (double)blk[c] * x_load(xc + c);
+    }
+    tp[slot][r][l] = acc;
+  }
+  __syncthreads();
+  if (act && l == 0) {
+    double sum = 0.0;
+#pragma unroll
+    for (int k = 0; k < LPR; ++k) sum += tp[slot][r][k];
+    v[slot][r] = (float)(forward ? bval - sum : sum);
+  }
+  __syncthreads();
+  if (act) {
+    float y = 0.0f;
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+      if (l + k * LPR < N) y += mrow[k] * v[slot][l + k * LPR];
+#pragma unroll
+    for (int k = PF; k < NC; ++k)
+      if (l + k * LPR < N) y += Mi[(l + k * LPR) * N + r] * v[slot][l + k * LPR];
+    yp[slot][r][l] = y;
+  }
+  __syncthreads();
+  if (act && l == 0) {
+    float y = 0.0f;
+#pragma unroll
+    for (int k = 0; k < LPR; ++k) y += yp[slot][r][k];
+    x_store(x + e * N + r, forward ? (double)y : bval - (double)y);
+  }
+  // this thread's stores have completed before the element's flag is set
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (act && tt == 0) __hip_atomic_store(&done[e], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // lanes per row of the sweeps: 1 for diagonal-E operators (a row meets at
 // most one neighbour node per face), up to 8 for dense face stencils
 template <int N, bool DIAGE>
@@ -501,8 +723,163 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
   }
   PDES_HIP(hipMemcpyAsync(p->d_fel, fel.data(), nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
   PDES_HIP(hipMemcpyAsync(p->d_bel, bel.data(), nel * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  {
+    // chunked, padded copies for the dependency-driven sweeps
+    const int C = ilu_dep_chunk(c->Np * c->nd, c->diag_e != 0);
+    auto pad = [&](const std::vector<int64_t>& ptr, const std::vector<int32_t>& el, std::vector<int32_t>& out) {
+      out.clear();
+      for (size_t lv = 0; lv + 1 < ptr.size(); ++lv) {
+        for (int64_t q = ptr[lv]; q < ptr[lv + 1]; ++q) out.push_back(el[q]);
+        while (out.size() % C) out.push_back(-1);
+      }
+    };
+    std::vector<int32_t> fpad, bpad;
+    pad(p->fptr, fel, fpad);
+    pad(p->bptr, bel, bpad);
+    p->chunk = C;
+    p->fpad_n = (int64_t)fpad.size() / C;
+    p->bpad_n = (int64_t)bpad.size() / C;
+    if (hipMalloc((void**)&p->d_fpad, std::max<size_t>(1, fpad.size()) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&p->d_bpad, std::max<size_t>(1, bpad.size()) * sizeof(int32_t)) != hipSuccess) {
+      set_error("device allocation of the ILU chunk lists failed");
+      return PDES_ERR_NOMEM;
+    }
+    PDES_HIP(hipMemcpyAsync(p->d_fpad, fpad.data(), fpad.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    PDES_HIP(hipMemcpyAsync(p->d_bpad, bpad.data(), bpad.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    PDES_HIP(hipStreamSynchronize(c->stream));   // the host vectors go out of scope
+  }
+  p->force = std::getenv("PDES_ILU_LEVELS") ? 1 : std::getenv("PDES_ILU_DEP") ? 2 : 0;
+  p->h_fl = fl;
+  p->h_bl = bl;
+  p->blk_per_el = (double)(J->nnzb - (int64_t)nel * c->Np * c->Np) / 2.0 / std::max(1, nel);
+  if (hipMalloc((void**)&p->d_done, std::max(1, nel) * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&p->d_sync, ILU_SYNC_WORDS * sizeof(int32_t)) != hipSuccess) {
+    set_error("device allocation of the ILU sweep flags failed");
+    return PDES_ERR_NOMEM;
+  }
+  PDES_HIP(hipMemsetAsync(p->d_done, 0, std::max(1, nel) * sizeof(int32_t), c->stream));
+  PDES_HIP(hipMemsetAsync(p->d_sync, 0, ILU_SYNC_WORDS * sizeof(int32_t), c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
   p->ilu_ready = true;
+  return PDES_OK;
+}
+
+// Which sweeps an application runs (both give the same bits).  A level
+// launch costs at least ILU_LEVEL_FLOOR_US and streams its elements' bytes at
+// ILU_LEVEL_BW; the dependency-driven sweep takes at least ILU_DEP_STEP_US per
+// level of its critical path, ILU_DEP_TICKET_US per chunk, and streams at
+// ILU_DEP_BW (measured on MI355X:
+// 2D diagonal-E p=2 sweeps over 246 levels of 123 elements take 4.8 us per
+// level launch and 2.6 us per dependency step; 3D Omega p=2 sweeps over 96
+// levels of 864 tets stream 2.1 TB/s level by level, 1.7 TB/s dependency-
+// driven).  The elements and levels are those of the instances being solved
+// when the batched solver says so, else the whole mesh.
+constexpr double ILU_LEVEL_FLOOR_US = 4.5, ILU_LEVEL_BW = 2.1e6;   // bytes per us
+constexpr double ILU_DEP_STEP_US = 2.6, ILU_DEP_BW = 1.7e6;
+// and one ticket (a device-scope atomic on one word) per chunk: about 21 ns
+// each, measured as the per-element rate of one-element chunks
+constexpr double ILU_DEP_TICKET_US = 0.021;
+
+static bool ilu_use_dep(const pdes_ctx* c, const pdes_precond* p) {
+  if (p->force) return p->force == 2;
+  const int nd = c->nd, n = c->Np * nd;
+  const double be = p->blk_per_el * nd * nd * (c->jac->fp64 ? 8.0 : 4.0) + (double)n * n * 4.0 + 3.0 * n * 8.0;
+  const bool hint = c->ilu_hint_el >= 0;
+  const double nel = hint ? (double)c->ilu_hint_el : (double)c->numEl;
+  double lev = 0.0, dep = 0.0;
+  for (int pass = 0; pass < 2; ++pass) {
+    const double L = std::max(1, hint ? (pass ? c->ilu_hint_lb : c->ilu_hint_lf)
+                                      : (int)(pass ? p->bptr.size() : p->fptr.size()) - 1);
+    lev += L * std::max(ILU_LEVEL_FLOOR_US, nel / L * be / ILU_LEVEL_BW);
+    dep += std::max(std::max(L * ILU_DEP_STEP_US, (nel / p->chunk + L) * ILU_DEP_TICKET_US), nel * be / ILU_DEP_BW);
+  }
+  return dep < lev;
+}
+
+struct ChunkSelected {
+  const int32_t* order;
+  int chunk;
+  const int32_t* el_inst;
+  const int32_t* sel;
+  __host__ __device__ bool operator()(const int32_t& ch) const {
+    for (int k = 0; k < chunk; ++k) {
+      const int32_t e = order[(int64_t)ch * chunk + k];
+      if (e >= 0 && sel[el_inst[e]]) return true;
+    }
+    return false;
+  }
+};
+
+// The batched solver's hint for the ILU(0) applications of its next Arnoldi
+// steps: they touch only the elements of the instances with inst_sel[i] != 0
+// (nel of them, over lf forward and lb backward levels).  When the cost model
+// then picks the dependency-driven sweeps, the level-ordered element lists are
+// compacted to those instances (order kept), so a sweep dispatches and
+// tickets only their elements.  ilu_hint_clear ends the hint.
+int ilu_hint(pdes_ctx* c, const int32_t* inst_sel, int ninst, int64_t nel, int lf, int lb) {
+  pdes_precond* p = c->pc_obj;
+  c->ilu_hint_el = nel;
+  c->ilu_hint_lf = lf;
+  c->ilu_hint_lb = lb;
+  c->ilu_sel = false;
+  if (!p || p->kind != PDES_PC_ILU0 || !c->d_el_inst || !ilu_use_dep(c, p)) return PDES_OK;
+  const int64_t nf = p->fpad_n, nb = p->bpad_n;
+  hipcub::CountingInputIterator<int32_t> ids(0);
+  if (!p->d_fsel) {
+    if (hipMalloc((void**)&p->d_fsel, std::max<int64_t>(1, nf) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&p->d_bsel, std::max<int64_t>(1, nb) * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&p->d_nsel, 2 * sizeof(int32_t)) != hipSuccess) {
+      set_error("device allocation of the ILU selection lists failed");
+      return PDES_ERR_NOMEM;
+    }
+    size_t bytes = 0;
+    PDES_HIP(hipcub::DeviceSelect::If(nullptr, bytes, ids, p->d_fsel, p->d_nsel, (int)std::max(nf, nb),
+                                      ChunkSelected{p->d_fpad, p->chunk, c->d_el_inst, nullptr}, c->stream));
+    if (hipMalloc(&p->d_select_tmp, std::max<size_t>(bytes, 16)) != hipSuccess) {
+      set_error("device allocation of the ILU selection work space failed");
+      return PDES_ERR_NOMEM;
+    }
+    p->select_tmp_bytes = bytes;
+  }
+  if (p->sel_ninst < ninst) {
+    if (p->d_inst_sel) hipFree(p->d_inst_sel);
+    p->d_inst_sel = nullptr;
+    if (hipMalloc((void**)&p->d_inst_sel, ninst * sizeof(int32_t)) != hipSuccess) {
+      set_error("device allocation of the ILU instance selection failed");
+      return PDES_ERR_NOMEM;
+    }
+    p->sel_ninst = ninst;
+  }
+  PDES_HIP(hipMemcpyAsync(p->d_inst_sel, inst_sel, ninst * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+  size_t bytes = p->select_tmp_bytes;
+  PDES_HIP(hipcub::DeviceSelect::If(p->d_select_tmp, bytes, ids, p->d_fsel, p->d_nsel, (int)nf,
+                                    ChunkSelected{p->d_fpad, p->chunk, c->d_el_inst, p->d_inst_sel}, c->stream));
+  bytes = p->select_tmp_bytes;
+  PDES_HIP(hipcub::DeviceSelect::If(p->d_select_tmp, bytes, ids, p->d_bsel, p->d_nsel + 1, (int)nb,
+                                    ChunkSelected{p->d_bpad, p->chunk, c->d_el_inst, p->d_inst_sel}, c->stream));
+  int32_t cnt[2] = {0, 0};
+  PDES_HIP(hipMemcpyAsync(cnt, p->d_nsel, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  p->sel_nf = cnt[0];
+  p->sel_nb = cnt[1];
+  c->ilu_sel = true;
+  return PDES_OK;
+}
+
+void ilu_hint_clear(pdes_ctx* c) {
+  c->ilu_hint_el = -1;
+  c->ilu_sel = false;
+}
+
+// levels spanned by the elements [el0, el1) (one instance of a packed batch)
+int ilu_range_levels(const pdes_ctx* c, int64_t el0, int64_t el1, int* lf, int* lb) {
+  const pdes_precond* p = c->pc_obj;
+  *lf = *lb = 0;
+  if (!p || p->h_fl.empty()) return PDES_ERR_ARG;
+  for (int64_t e = el0; e < el1; ++e) {
+    *lf = std::max(*lf, p->h_fl[e] + 1);
+    *lb = std::max(*lb, p->h_bl[e] + 1);
+  }
   return PDES_OK;
 }
 
@@ -527,13 +904,41 @@ static int ilu_factor(pdes_ctx* c) {
   return PDES_OK;
 }
 
-template <int NP, int ND, bool DIAGE>
+template <int NP, int ND, int NS, bool DIAGE>
 static int ilu_apply(pdes_ctx* c, const double* b, double* x) {
   pdes_jacobian* j = c->jac;
   pdes_precond* p = c->pc_obj;
   constexpr int N = NP * ND;
   constexpr int LPR = ilu_lanes<N, DIAGE>();
   constexpr int EPB = 256 / (N * LPR);
+  if (ilu_use_dep(c, p)) {
+    constexpr int CH = ilu_dep_chunk(N, DIAGE);
+    static_assert(ilu_dep_lanes(N, DIAGE) == LPR, "the dependency-driven and level sweeps split rows alike");
+    if (p->chunk != CH) {
+      set_error("ILU chunk size mismatch");
+      return PDES_ERR_ARG;
+    }
+    const bool sel = c->ilu_sel && c->ilu_hint_el >= 0;
+    for (int pass = 0; pass < 2; ++pass) {
+      const int32_t* order = pass == 0 ? p->d_fpad : p->d_bpad;
+      const int32_t* chunks = sel ? (pass == 0 ? p->d_fsel : p->d_bsel) : nullptr;
+      const int64_t nch = sel ? (pass == 0 ? p->sel_nf : p->sel_nb) : (pass == 0 ? p->fpad_n : p->bpad_n);
+      if (nch == 0) continue;
+      hipLaunchKernelGGL(ilu_sweep_begin, dim3(1), dim3(1), 0, c->stream, p->d_sync);
+      if (j->fp64)
+        hipLaunchKernelGGL((ilu_sweep_dep_kernel<NP, ND, NS, DIAGE, double, LPR, CH>), dim3((unsigned)nch),
+                           dim3(CH * N * LPR), 0, c->stream, order, chunks, nch, j->d_rowptr, j->d_colind, p->d_own,
+                           (const double*)j->d_vals, p->d_inv, b, x, pass == 0, c->d_el_inst, c->d_mask,
+                           c->d_inst_active, c->d_elface, c->nfaces, p->d_done, p->d_sync);
+      else
+        hipLaunchKernelGGL((ilu_sweep_dep_kernel<NP, ND, NS, DIAGE, float, LPR, CH>), dim3((unsigned)nch),
+                           dim3(CH * N * LPR), 0, c->stream, order, chunks, nch, j->d_rowptr, j->d_colind, p->d_own,
+                           (const float*)j->d_vals, p->d_inv, b, x, pass == 0, c->d_el_inst, c->d_mask,
+                           c->d_inst_active, c->d_elface, c->nfaces, p->d_done, p->d_sync);
+    }
+    PDES_HIP(hipGetLastError());
+    return PDES_OK;
+  }
   for (int pass = 0; pass < 2; ++pass) {
     const std::vector<int64_t>& ptr = pass == 0 ? p->fptr : p->bptr;
     const int32_t* els = pass == 0 ? p->d_fel : p->d_bel;
@@ -632,7 +1037,7 @@ static int ilu_factor_dispatch(pdes_ctx* c) {
 
 static int ilu_apply_dispatch(pdes_ctx* c, const double* b, double* x) {
 #define PDES_ILU_A(D_, NP_, NF_, NS_, DE_) \
-  if (c->dim == D_ && c->Np == NP_ && (c->diag_e != 0) == DE_) return ilu_apply<NP_, D_ + 2, DE_>(c, b, x);
+  if (c->dim == D_ && c->Np == NP_ && (c->diag_e != 0) == DE_) return ilu_apply<NP_, D_ + 2, NS_, DE_>(c, b, x);
   PDES_FOR_EACH_SHAPE(PDES_ILU_A)
 #undef PDES_ILU_A
   set_error("no ILU(0) kernel for this operator");
@@ -683,8 +1088,10 @@ static int ilu_apply_graphed(pdes_ctx* c, const double* b, double* x) {
 }
 
 int pc_apply(pdes_ctx* c, const double* b, double* x) {
+  // the dependency-driven sweeps are 4 launches; the level-by-level ones are
+  // replayed from a captured graph
   if (c->pc_obj && c->pc_obj->kind == PDES_PC_ILU0)
-    return c->use_graphs ? ilu_apply_graphed(c, b, x) : ilu_apply_dispatch(c, b, x);
+    return c->use_graphs && !ilu_use_dep(c, c->pc_obj) ? ilu_apply_graphed(c, b, x) : ilu_apply_dispatch(c, b, x);
   if (!c->pc_obj || c->pc_obj->kind == PDES_PC_NONE) {
     const int64_t n = c->numDof;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)std::min<int64_t>(8192, ceil_div(n, 256))), dim3(256), 0,
@@ -711,6 +1118,22 @@ int pc_apply(pdes_ctx* c, const double* b, double* x) {
   }
   set_error("no block-Jacobi kernel for this operator");
   return PDES_ERR_UNSUPPORTED;
+}
+
+// the error word of the dependency-driven sweeps (a wait that gave up): the
+// preconditioner applications since the last check are not to be trusted
+int pc_check(pdes_ctx* c) {
+  pdes_precond* p = c->pc_obj;
+  if (!p || p->kind != PDES_PC_ILU0 || !p->d_sync) return PDES_OK;
+  int32_t err = 0;
+  PDES_HIP(hipMemcpyAsync(&err, p->d_sync + ILU_SYNC_ERR, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  PDES_HIP(hipStreamSynchronize(c->stream));
+  if (err) {
+    PDES_HIP(hipMemsetAsync(p->d_sync + ILU_SYNC_ERR, 0, sizeof(int32_t), c->stream));
+    set_error("an ILU(0) sweep gave up waiting for a neighbour element (its result is invalid)");
+    return PDES_ERR_HIP;
+  }
+  return PDES_OK;
 }
 
 int pc_setup(pdes_ctx* c, int kind) {
@@ -992,6 +1415,7 @@ int gmres_device(pdes_ctx* c, const double* b, double* x, double rtol, double at
 done:
   gmres_keep(c, W);
   if (st) return st;
+  if ((st = pc_check(c)) != 0) return st;
   if (iters) *iters = total;
   if (resnorm) *resnorm = beta;
   (void)converged;
@@ -1028,7 +1452,7 @@ int pdes_pc_apply(pdes_ctx* c, const double* b, double* x) {
   if (st) return st;
   PDES_HIP(hipMemcpyAsync(x, c->d_work2, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
-  return PDES_OK;
+  return pc_check(c);
 }
 
 int pdes_gmres(pdes_ctx* c, const double* b_host, double* x_host, double rtol, double atol, int restart,

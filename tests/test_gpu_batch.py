@@ -154,3 +154,34 @@ def test_repack_mid_solve_is_bit_identical(pc_kind, globalize):
         assert r.converged == 1 and r.res_norm <= 1e-10
         assert r.iterations == ref_reps[i].iterations and r.total_krylov_iters == ref_reps[i].total_krylov_iters, i
         assert np.array_equal(q, ref_q[i]), f"instance {i} differs after the move"
+
+
+def test_batch_ilu_sweep_kinds_bit_identical(monkeypatch):
+    """A batched ILU(0) solve of uneven instances (diagonal-E p=2 and p=3,
+    Omega p=2) gives the same bits, Newton and Krylov counts with the
+    dependency-driven sweeps (PDES_ILU_DEP: chunks of same-level elements in
+    ticket order, compacted to the instances still solving at each GMRES
+    cycle) as with the level-by-level sweeps (PDES_ILU_LEVELS)."""
+    opts = linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-2, itermax=30, pc_kind=2)
+    probs = []
+    for i, (n, p, opk, flux) in enumerate([(14, 2, "diage", "ir"), (6, 2, "diage", "ir"), (9, 3, "diage", "ir"),
+                                           (8, 2, "omega", "roe"), (11, 2, "diage", "ir")]):
+        m, op, ph = problems.make_problem("vortex", n, p, opk, flux, vortex_Min=0.4 + 0.05 * i)
+        probs.append((m, op, ph, problems.vortex_noise_state(m, ph, seed=40 + i)))
+    out = {}
+    for kind in ("PDES_ILU_DEP", "PDES_ILU_LEVELS"):
+        monkeypatch.setenv(kind, "1")
+        monkeypatch.delenv("PDES_ILU_LEVELS" if kind == "PDES_ILU_DEP" else "PDES_ILU_DEP", raising=False)
+        bs = BatchSolver([(m, op, ph) for (m, op, ph, q) in probs])
+        for i, p in enumerate(probs):
+            bs.set_state(i, p[3])
+        reps = bs.solve(opts)
+        out[kind] = ([bs.get_state(i) for i in range(len(probs))],
+                     [(r.converged, r.iterations, r.total_krylov_iters) for r in reps])
+        bs.free()
+        print(kind, out[kind][1], flush=True)
+    (qd, rd), (ql, rl) = out["PDES_ILU_DEP"], out["PDES_ILU_LEVELS"]
+    assert rd == rl
+    assert all(r[0] == 1 for r in rd)
+    for i in range(len(probs)):
+        assert np.array_equal(qd[i], ql[i]), f"instance {i} differs between the sweep kinds"

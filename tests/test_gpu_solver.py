@@ -360,13 +360,14 @@ def test_newton_with_ilu0(kind, n, p, opk, flux):
 
 
 def test_ilu0_graph_replays_match_eager(monkeypatch):
-    """The ILU(0) apply replays a captured HIP graph of its level launches.
-    Every replay, with other device work and synchronisations in between,
-    gives the same bits as the directly launched sweeps of a context created
-    with PDES_NO_GRAPHS."""
+    """The level-by-level ILU(0) apply (PDES_ILU_LEVELS) replays a captured
+    HIP graph of its level launches.  Every replay, with other device work and
+    synchronisations in between, gives the same bits as the directly launched
+    sweeps of a context created with PDES_NO_GRAPHS."""
     m, op, ph = problems.make_problem("vortex", 6, 3, "diage", "ir")
     q = problems.vortex_noise_state(m, ph, seed=5)
     ctxs = []
+    monkeypatch.setenv("PDES_ILU_LEVELS", "1")
     for eager in (False, True):
         if eager:
             monkeypatch.setenv("PDES_NO_GRAPHS", "1")
@@ -388,6 +389,51 @@ def test_ilu0_graph_replays_match_eager(monkeypatch):
             pc.applyPC(m, op, e, None, 0.0, b, x)
             out.append(x)
         assert np.array_equal(out[0], out[1]), f"replay {k} differs from the eager sweeps"
+    for e, pc in ctxs:
+        e.free()
+
+
+DEP_CASES = [("vortex", 6, 2, "diage", "ir", False), ("vortex", 5, 3, "diage", "ir", True),
+             ("vortex", 4, 4, "diage", "ir", False), ("vortex", 5, 2, "omega", "roe", False),
+             ("vortex", 4, 1, "omega", "roe", True), ("cube", 2, 2, "omega", "roe", False),
+             ("cube", 2, 1, "omega", "roe", False), ("vortex", 128, 2, "diage", "roe", False)]
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux,fp64", DEP_CASES)
+def test_ilu0_dependency_sweeps_match_level_sweeps(monkeypatch, kind, n, p, opk, flux, fp64):
+    """The dependency-driven ILU(0) sweeps (one launch per triangle, elements
+    in ticket order, each waiting for its lower or upper neighbours'
+    completion flags; forced with PDES_ILU_DEP) give the bits of the
+    level-by-level sweeps (forced with PDES_ILU_LEVELS), over repeated
+    applications (the flags' epochs advance, nothing is cleared) with other
+    device work in between, for one-lane rows (diagonal-E) and several lanes
+    per row (dense faces), FP32 and FP64 blocks, up to 128 x 128 x 2
+    triangles.  Unforced, a cost model picks one of the two per application,
+    so either choice gives the same result."""
+    print(f"case {kind} n={n} p={p} {opk} {flux} fp64={fp64}", flush=True)
+    m, op, ph = problems.make_problem(kind, n, p, opk, flux, Ma=0.5, aoa_deg=2.0)
+    q = problems.vortex_noise_state(m, ph, seed=5) if kind == "vortex" else problems.freestream_bump_state(m, ph)
+    ctxs = []
+    for levels in (False, True):
+        monkeypatch.setenv("PDES_ILU_LEVELS" if levels else "PDES_ILU_DEP", "1")
+        monkeypatch.delenv("PDES_ILU_DEP" if levels else "PDES_ILU_LEVELS", raising=False)
+        e = euler.EulerData(m, op, ph)
+        e.set_q(q)
+        linsolve.evalJacobian(m, op, e, fp64_blocks=fp64)
+        pc = linsolve.HIPILU0PC(e)
+        pc.calcPC()
+        ctxs.append((e, pc))
+    rng = np.random.default_rng(13)
+    for k in range(3):
+        b = rng.standard_normal(q.shape)
+        out = []
+        for e, pc in ctxs:
+            euler.evalResidual(m, op, e)
+            x = np.zeros_like(b)
+            pc.applyPC(m, op, e, None, 0.0, b, x)
+            out.append(x)
+        assert np.all(np.isfinite(out[0]))
+        assert np.array_equal(out[0], out[1]), f"application {k}: dependency sweeps differ from level sweeps"
     for e, pc in ctxs:
         e.free()
 
