@@ -13,6 +13,7 @@ Both paths build their device objects before the timed region; the timed
 region is state upload + Newton solve (+ nothing else) for every instance.
 
     python pdesolver.jl_amd/bench_solves.py [--count 96] [--nmax 128]
+    python pdesolver.jl_amd/bench_solves.py --single 18 [--steps 1]   (one instance alone)
 """
 from __future__ import annotations
 
@@ -136,8 +137,38 @@ def run(count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, check_to
     }
 
 
+def run_single(index: int, count: int = 96, seed: int = 3, nmin: int = 8, nmax: int = 128, pc: str = "ilu0",
+               restart: int = 30, steps: int = 50):
+    """One instance of the workload solved alone on a single context, one
+    Newton step per call so that each step prints (at most `steps` steps)."""
+    inst = problems.config3_instances(count=count, seed=seed, nmin=nmin, nmax=nmax)
+    m, op, ph, q0 = inst[index]
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q0)
+    opts = newton_options(pc, restart)
+    opts.itermax = 1
+    t0 = time.perf_counter()
+    its = kry = 0
+    rep = None
+    for _ in range(steps):
+        rep = linsolve.newton(m, op, eqn, opts)
+        its += rep.iterations
+        kry += rep.total_krylov_iters
+        print(f"instance {index} step {its}: ||R|| {rep.res_norm:.3e}, {kry} Krylov, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+        if rep.converged:
+            break
+    t = time.perf_counter() - t0
+    eqn.free()
+    return {"instance": index, "operator": f"{op.kind} p={op.degree}", "numEl": int(m.numEl),
+            "newton_iterations": its, "krylov_iterations": kry, "converged": int(rep.converged),
+            "res_norm": rep.res_norm, "solve_s": t}
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--single", type=int, default=-1, help="solve only this instance, alone, step by step")
+    ap.add_argument("--steps", type=int, default=50, help="with --single: at most this many Newton steps")
     ap.add_argument("--count", type=int, default=96)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--nmin", type=int, default=8)
@@ -146,6 +177,9 @@ def main():
     ap.add_argument("--pc", choices=sorted(PCS), default="ilu0")
     ap.add_argument("--restart", type=int, default=30)
     a = ap.parse_args()
+    if a.single >= 0:
+        print(json.dumps(run_single(a.single, a.count, a.seed, a.nmin, a.nmax, a.pc, a.restart, a.steps)), flush=True)
+        return
     print(json.dumps(run(a.count, a.seed, a.nmin, a.nmax, only=a.only, pc=a.pc, restart=a.restart)), flush=True)
 
 

@@ -42,31 +42,48 @@ __device__ __forceinline__ int64_t active_dof(const int32_t* el_list, int64_t t,
   return (int64_t)el_list[s] * dpe + (t - s * dpe);
 }
 
-// partial[c][i] = sum over chunk c of V_i . w  (w2: optional weights)
+// partial[c][i] = sum over chunk c of V_i . w  (wt: optional weights, then
+// sum of V_i^2 wt).  SEG_GROUP basis vectors per pass over the chunk, so w is
+// read once per group; each vector's terms are summed in the same order, and
+// reduced by the same tree, as one vector at a time would be.
+constexpr int SEG_GROUP = 8;
 __global__ void __launch_bounds__(SEG_THREADS)
 seg_dot_partial(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ w,
                 const double* __restrict__ wt, const int64_t* __restrict__ cbeg, const int32_t* __restrict__ clen,
                 const int32_t* __restrict__ clist, double* __restrict__ partial, int mstride,
                 const int32_t* __restrict__ chunk_inst, const int32_t* __restrict__ mask) {
-  __shared__ double sh[SEG_THREADS];
+  __shared__ double sh[SEG_GROUP][SEG_THREADS];
   const int c = clist[blockIdx.x];
   if (mask && !mask[chunk_inst[c]]) return;
   const int64_t b0 = cbeg[c];
   const int len = clen[c];
-  for (int i = 0; i < m; ++i) {
-    const double* Vi = V + (int64_t)i * ldv;
-    double acc = 0.0;
+  for (int i0 = 0; i0 < m; i0 += SEG_GROUP) {
+    const int g_n = m - i0 < SEG_GROUP ? m - i0 : SEG_GROUP;
+    double acc[SEG_GROUP];
+#pragma unroll
+    for (int g = 0; g < SEG_GROUP; ++g) acc[g] = 0.0;
     for (int t = threadIdx.x; t < len; t += SEG_THREADS) {
       const int64_t k = b0 + t;
-      acc += wt ? Vi[k] * Vi[k] * wt[k] : Vi[k] * w[k];
+      const double wk = wt ? wt[k] : w[k];
+#pragma unroll
+      for (int g = 0; g < SEG_GROUP; ++g) {
+        if (g < g_n) {
+          const double v = V[(int64_t)(i0 + g) * ldv + k];
+          acc[g] += wt ? v * v * wk : v * wk;
+        }
+      }
     }
-    sh[threadIdx.x] = acc;
+#pragma unroll
+    for (int g = 0; g < SEG_GROUP; ++g) sh[g][threadIdx.x] = acc[g];
     __syncthreads();
     for (int s = SEG_THREADS / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      if ((int)threadIdx.x < s) {
+#pragma unroll
+        for (int g = 0; g < SEG_GROUP; ++g) sh[g][threadIdx.x] += sh[g][threadIdx.x + s];
+      }
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(int64_t)c * mstride + i] = sh[0];
+    if ((int)threadIdx.x < g_n) partial[(int64_t)c * mstride + i0 + threadIdx.x] = sh[threadIdx.x][0];
     __syncthreads();
   }
 }

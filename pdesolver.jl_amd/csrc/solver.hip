@@ -1173,23 +1173,38 @@ int pc_setup(pdes_ctx* c, int kind) {
 constexpr int MD_THREADS = 256;
 constexpr int MD_BLOCKS = 512;
 
-// partial[i][b] = sum over block b's slice of V_i . w
+// partial[i][b] = sum over block b's slice of V_i . w.  MD_GROUP basis
+// vectors per pass over the slice, so w is read once per group; each vector's
+// terms are summed in the same order, and reduced by the same tree, as one
+// vector at a time would be.
+constexpr int MD_GROUP = 8;
 __global__ void __launch_bounds__(MD_THREADS)
 multidot_partial(const double* __restrict__ V, int m, int64_t ldv, const double* __restrict__ w, int64_t n,
                  double* __restrict__ partial) {
-  __shared__ double sh[MD_THREADS];
+  __shared__ double sh[MD_GROUP][MD_THREADS];
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int i = 0; i < m; ++i) {
-    double acc = 0.0;
-    const double* Vi = V + (int64_t)i * ldv;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) acc += Vi[t] * w[t];
-    sh[threadIdx.x] = acc;
+  for (int i0 = 0; i0 < m; i0 += MD_GROUP) {
+    const int g_n = m - i0 < MD_GROUP ? m - i0 : MD_GROUP;
+    double acc[MD_GROUP];
+#pragma unroll
+    for (int g = 0; g < MD_GROUP; ++g) acc[g] = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+      const double wt = w[t];
+#pragma unroll
+      for (int g = 0; g < MD_GROUP; ++g)
+        if (g < g_n) acc[g] += V[(int64_t)(i0 + g) * ldv + t] * wt;
+    }
+#pragma unroll
+    for (int g = 0; g < MD_GROUP; ++g) sh[g][threadIdx.x] = acc[g];
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      if ((int)threadIdx.x < s) {
+#pragma unroll
+        for (int g = 0; g < MD_GROUP; ++g) sh[g][threadIdx.x] += sh[g][threadIdx.x + s];
+      }
       __syncthreads();
     }
-    if (threadIdx.x == 0) partial[(int64_t)i * gridDim.x + blockIdx.x] = sh[0];
+    if ((int)threadIdx.x < g_n) partial[(int64_t)(i0 + threadIdx.x) * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
     __syncthreads();
   }
 }
