@@ -310,9 +310,10 @@ int pdes_homotopy_solve(pdes_ctx* ctx, const pdes_newton_opts* opts, double psi_
                         pdes_newton_report* report);
 
 /* Boundary force functionals (calcBndryFunctional,
- * src/solver/euler/boundary_functional.jl:82-126):
+ * src/solver/euler/boundary_functional.jl:82-126, functionals.jl:19-160):
  * kind 0 = force vector (dim values), 1 = lift, 2 = drag, 3 = entropy flux,
- * 4 = mass flow; over the BC groups listed in bcnums. */
+ * 4 = mass flow, 5 = lift coefficient, 6 = drag coefficient (lift and drag
+ * over 0.5 rho_free Ma^2); over the BC groups listed in bcnums. */
 int pdes_eval_functional(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
                          double* val);
 
@@ -322,6 +323,29 @@ int pdes_eval_functional(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc
  * with respect to q at the current state, in the layout of q. */
 int pdes_functional_deriv(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
                           double* dJdq);
+
+/* evalFunctionalDeriv_m (src/interface2.jl:205-227 -> _evalFunctionalDeriv_m,
+ * src/solver/euler/boundary_functional.jl:61-79): the derivative of the scalar
+ * functional `kind` (1..6) with respect to the boundary normals,
+ * nrm_bndry_bar[numBoundaryFaces][Nf][dim] (the layout of nrm_bndry; zero on
+ * faces outside bcnums), overwritten.  The functionals depend on no other
+ * metric array. */
+int pdes_functional_deriv_m(pdes_ctx* ctx, int kind, const int32_t* bcnums, int nbc,
+                            double* nrm_bndry_bar);
+
+/* evalResidual_revm (src/interface2.jl:329-372 -> src/solver/euler/evaldRdm.jl:
+ * 19-70): with the seed psi[numDof] (res_bar, host), the products
+ * psi^T dR/dm at the context's state for the mesh metrics, overwritten:
+ *   dxidx_bar[numEl][Np][dim][dim]          (the layout of dxidx),
+ *   nrm_face_bar[numInterfaces][Nf][dim]    (the layout of nrm_face),
+ *   nrm_bndry_bar[numBoundaryFaces][Nf][dim],
+ *   coords_bndry_bar[numBoundaryFaces][Nf][dim]  (through the boundary states).
+ * The reference adds into the mesh's _bar fields, which its caller zeroes
+ * first.  Any output may be NULL.  Weak form with Roe faces and split form
+ * with IRSLF faces; PDES_ERR_UNSUPPORTED for the entropy-stable face integral
+ * (face_flux ESLF) and for packed batches. */
+int pdes_eval_residual_revm(pdes_ctx* ctx, const double* psi, double* dxidx_bar, double* nrm_face_bar,
+                            double* nrm_bndry_bar, double* coords_bndry_bar);
 
 /* applyLinearOperatorTranspose (src/linearsolvers/LinearSolvers.jl:533-541,
  * the matrix-explicit method src/linearsolvers/lo_petscmat.jl:143-161):

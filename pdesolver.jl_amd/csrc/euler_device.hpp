@@ -26,8 +26,8 @@ __device__ __forceinline__ T pressure(const PhysConst& p, const T* q) {
 }
 
 // calcEulerFlux  euler_funcs.jl:512-536 (2D), 749-774 (3D)
-template <int DIM, class T>
-__device__ __forceinline__ void euler_flux(const PhysConst& p, const T* q, const double* dir, T* F) {
+template <int DIM, class T, class TN = double>
+__device__ __forceinline__ void euler_flux(const PhysConst& p, const T* q, const TN* dir, T* F) {
   T press = pressure<DIM>(p, q);
   if constexpr (DIM == 2) {
     T U = (q[1] * dir[0] + q[2] * dir[1]) * drcp(q[0]);
@@ -47,8 +47,10 @@ __device__ __forceinline__ void euler_flux(const PhysConst& p, const T* q, const
 
 // RoeSolver + calcSAT  src/solver/euler/bc_solvers.jl:29-103 / 207-310 (2D),
 // 111-187 / 313-420 (3D).  flux = SAT(qL - qR) + F_euler(qL).n
-template <int DIM, class T>
-__device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T* qg, const double* nrm, T* flux) {
+// (TN: the normal's type, double, or the dual type of T for derivatives with
+// respect to the normal)
+template <int DIM, class T, class TN = double>
+__device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T* qg, const TN* nrm, T* flux) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gami = p.gamma_1;
   const double sat_Vn = 0.025, sat_Vl = 0.025, tau = 1.0;
@@ -79,8 +81,8 @@ __device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T
 #pragma unroll
   for (int i = 0; i < ND; ++i) dq[i] = q[i] - qg[i];
   // calcSAT
-  double nx = nrm[0], ny = nrm[1], nz = 0.0;
-  double dA;
+  TN nx = nrm[0], ny = nrm[1], nz = TN(0.0);
+  TN dA;
   T phi, Un;
   if constexpr (DIM == 2) {
     dA = dsqrt(nx * nx + ny * ny);
@@ -105,7 +107,7 @@ __device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T
   for (int i = 0; i < ND; ++i) sat[i] = lambda3 * dq[i];
   T E1dq[ND], E2dq[ND];
   T vel[3] = {u, v, w};
-  double n3[3] = {nx, ny, nz};
+  TN n3[3] = {nx, ny, nz};
   // E1 dq, E2 dq
   {
     T e1 = phi * dq[0] - u * dq[1] - v * dq[2];
@@ -123,7 +125,7 @@ __device__ __forceinline__ void roe_flux(const PhysConst& p, const T* q, const T
     E2dq[ND - 1] = e2 * Un;
     T tmp1 = 0.5 * (lambda1 + lambda2) - lambda3;
     T tmp2 = gami * drcp(a * a);
-    double tmp3 = drcp(dA * dA);
+    TN tmp3 = drcp(dA * dA);
 #pragma unroll
     for (int i = 0; i < ND; ++i) sat[i] = sat[i] + tmp1 * (tmp2 * E1dq[i] + tmp3 * E2dq[i]);
     // E3 dq, E4 dq
@@ -191,9 +193,9 @@ __device__ __forceinline__ T logavg_factor(const T& aL, const T& aR, const T& r,
 // log z1, with log z4 LS elements further; read only on the log branch of
 // logavg), or nullptr.  dl: optional output {log(zL1/zR1), log(zL4/zR4)},
 // which the logavg factors give without a log: log(aL/aR) = 2 F f.
-template <int DIM, int NDIR, class T, class LP = std::nullptr_t, int LS = 1>
+template <int DIM, int NDIR, class T, class LP = std::nullptr_t, int LS = 1, class TN = double>
 __device__ __forceinline__ void ir_flux_z(const PhysConst& p, const T* zL, const T* zR,
-                                          const double* dir, T* F, LP lzL = nullptr, LP lzR = nullptr,
+                                          const TN* dir, T* F, LP lzL = nullptr, LP lzR = nullptr,
                                           T* dl = nullptr) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
@@ -271,8 +273,8 @@ __device__ __forceinline__ void to_ir_vars(const PhysConst& p, const T* qc, T* q
 //   row d:    m_d (dv_0 + s/rho + h dv_E) + p dv_d
 //   row E:    E dv_0 + h s + (rho h^2 - a^2 p/(gamma-1)) dv_E
 // and getLambdaMax (euler_funcs.jl:1887-1910) = |U_n| + |n| a.
-template <int DIM, class T>
-__device__ __forceinline__ void lf_entropy_apply(const PhysConst& p, const T* qa, const T* dv, const double* nrm,
+template <int DIM, class T, class TN = double>
+__device__ __forceinline__ void lf_entropy_apply(const PhysConst& p, const T* qa, const T* dv, const TN* nrm,
                                                  double scale, T* F) {
   constexpr int ND = DIM + 2;
   const double gamma = p.gamma, gamma_1 = p.gamma_1;
@@ -282,11 +284,11 @@ __device__ __forceinline__ void lf_entropy_apply(const PhysConst& p, const T* qa
   T h = (qa[ND - 1] + pr) * rhoinv;
   T a2 = gamma * pr * rhoinv;
   T Un = 0.0;
-  double dA = 0.0;
+  TN dA = 0.0;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
     Un = Un + nrm[i] * qa[i + 1] * rhoinv;
-    dA += nrm[i] * nrm[i];
+    dA = dA + nrm[i] * nrm[i];
   }
   dA = dsqrt(dA);
   const T lambda_max = scale * (absvalue3(Un) + dA * dsqrt(a2));
@@ -304,9 +306,9 @@ __device__ __forceinline__ void lf_entropy_apply(const PhysConst& p, const T* qa
 // IRSLF face flux: calcEulerFlux_IRSLF bc_solvers.jl:898-909 ->
 // applyEntropyKernel_diagE faceElementIntegrals.jl:510-575 -> LFKernel :455-470
 // with getIRA0 IR_stab.jl:15-110 and getLambdaMax euler_funcs.jl:1887-1910.
-template <int DIM, class T>
+template <int DIM, class T, class TN = double>
 __device__ __forceinline__ void irslf_flux(const PhysConst& p, const T* qL, const T* qR,
-                                           const double* nrm, T* F) {
+                                           const TN* nrm, T* F) {
   constexpr int ND = DIM + 2;
   T zL[ND], zR[ND];
   ir_zvars<DIM>(p, qL, zL);

@@ -18,18 +18,29 @@
 
 namespace pdes {
 
+// 1/sqrt(x), for the normal's length
+__device__ __forceinline__ double inv_len(double x) { return 1.0 / sqrt(x); }
+template <int N> __device__ __forceinline__ dual<N> inv_len(const dual<N>& a) {
+  dual<N> r;
+  r.v = 1.0 / sqrt(a.v);
+  const double f = -0.5 * r.v / a.v;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * f;
+  return r;
+}
+
 // the five integrands at one face node: force x, y, z (weighted by wface),
-// entropy flux, mass flow; T is double, or dual<ND> for their derivatives
-// with respect to the face-node state
-template <int DIM, class T>
-__device__ __forceinline__ void face_integrands(const PhysConst& phys, const T* qb, const double* nrm, double w,
+// entropy flux, mass flow; T is double, or a dual type for their derivatives
+// with respect to the face-node state (TN double) or to the normal (TN = T)
+template <int DIM, class T, class TN = double>
+__device__ __forceinline__ void face_integrands(const PhysConst& phys, const T* qb, const TN* nrm, double w,
                                                 T* o) {
   constexpr int ND = DIM + 2;
   // force
-  double nn = 0.0;
-  for (int d = 0; d < DIM; ++d) nn += nrm[d] * nrm[d];
-  const double fac = 1.0 / sqrt(nn);
-  double nh[DIM];
+  TN nn = 0.0;
+  for (int d = 0; d < DIM; ++d) nn = nn + nrm[d] * nrm[d];
+  const TN fac = inv_len(nn);
+  TN nh[DIM];
   T normal_momentum = 0.0;
   for (int d = 0; d < DIM; ++d) { nh[d] = nrm[d] * fac; normal_momentum = normal_momentum + nh[d] * qb[1 + d]; }
   T qg[ND];
@@ -107,6 +118,42 @@ __global__ void functional_face_grad_kernel(DevMesh mesh, DevSBP sbp, PhysConst 
   }
 }
 
+// evalFunctionalDeriv_m (src/interface2.jl:205-227 -> _evalFunctionalDeriv_m,
+// boundary_functional.jl:61-79) for a scalar functional sum_k wts .
+// integrands_k: the integrands depend on the metrics through the boundary
+// normals only, so nrm_bndry_bar[b][k][x] = sum_v wts[v] d(integrand_v)/dn_x
+// (forward mode over the normal) at the selected faces, zero elsewhere
+template <int DIM, int NF, int ST>
+__global__ void functional_nrm_grad_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const double* __restrict__ q,
+                                           int Np, const int32_t* __restrict__ bfaces, int nbndry,
+                                           const int32_t* __restrict__ sel, const double* __restrict__ wts,
+                                           double* __restrict__ nbar) {
+  constexpr int ND = DIM + 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nbndry * NF) return;
+  const int b = t / NF, k = t - b * NF;
+  double* gk = nbar + (int64_t)t * DIM;
+  if (!sel[b]) {
+    for (int x = 0; x < DIM; ++x) gk[x] = 0.0;
+    return;
+  }
+  double qv[ND];
+  face_state<DIM, NF, ST>(sbp, q, Np, bfaces[2 * b], bfaces[2 * b + 1], k, qv);
+  dual<DIM> qb[ND], nd[DIM], o[5];
+  for (int c = 0; c < ND; ++c) qb[c] = dual<DIM>(qv[c]);
+  const double* nr = mesh.nrm_bndry + ((int64_t)b * NF + k) * DIM;
+  for (int x = 0; x < DIM; ++x) {
+    nd[x] = dual<DIM>(nr[x]);
+    nd[x].d[x] = 1.0;
+  }
+  face_integrands<DIM>(phys, qb, nd, sbp.wface[k], o);
+  for (int x = 0; x < DIM; ++x) {
+    double g = 0.0;
+    for (int v = 0; v < 5; ++v) g += wts[v] * o[v].d[x];
+    gk[x] = g;
+  }
+}
+
 // dJ/dq at volume node i of element e: sum over the element's boundary faces f
 // that contain node i (stencil position s) of sum_k R[s][k] g_k; one thread
 // per (element, node), so every entry has one writer
@@ -134,21 +181,21 @@ __global__ void functional_gather_kernel(DevMesh mesh, DevSBP sbp, int Np, const
   for (int c = 0; c < ND; ++c) dJdq[t * ND + c] = acc[c];
 }
 
-// dJ/dq of the scalar functional `kind` (1..6) over the BC groups bcnums,
-// into the device vector d_out [numDof]
-int functional_grad_device(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* d_out) {
-  PDES_CHECK(c && bcnums && d_out && nbc > 0 && kind >= 1 && kind <= 6, PDES_ERR_ARG,
+// the boundary faces of the BC groups bcnums (sel[b] = 1) and the weights of
+// the five integrands that make up the scalar functional `kind` (1..6)
+static int functional_setup(const pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, std::vector<int32_t>& sel,
+                            double* wts) {
+  PDES_CHECK(c && bcnums && nbc > 0 && kind >= 1 && kind <= 6, PDES_ERR_ARG,
              "bad argument (the derivative needs a scalar functional, kind 1..6)");
-  const int nb = (int)(c->h_bndryfaces.size() / 2), Nf = c->Nf, dim = c->dim;
-  std::vector<int32_t> sel(std::max(1, nb), 0);
+  const int nb = (int)(c->h_bndryfaces.size() / 2);
+  sel.assign(std::max(1, nb), 0);
   for (int i = 0; i < nbc; ++i) {
     PDES_CHECK(bcnums[i] >= 0 && bcnums[i] < c->numBC, PDES_ERR_ARG, "BC number %d out of range", bcnums[i]);
     for (int b = c->h_bndry_offsets[bcnums[i]]; b < c->h_bndry_offsets[bcnums[i] + 1]; ++b) sel[b] = 1;
   }
-  // weights of the five integrands that make up the functional
   const double aoa = c->phys.aoa, qdyn = 0.5 * c->phys.rho_free * c->phys.Ma * c->phys.Ma;
-  const int up = dim == 2 ? 1 : 2;
-  double wts[5] = {0, 0, 0, 0, 0};
+  const int up = c->dim == 2 ? 1 : 2;
+  for (int v = 0; v < 5; ++v) wts[v] = 0.0;
   switch (kind) {
     case 1: case 5: wts[0] = -std::sin(aoa); wts[up] = std::cos(aoa); break;
     case 2: case 6: wts[0] = std::cos(aoa); wts[up] = std::sin(aoa); break;
@@ -156,7 +203,19 @@ int functional_grad_device(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc
     case 4: wts[4] = 1.0; break;
   }
   if (kind >= 5)
-    for (double& w : wts) w /= qdyn;
+    for (int v = 0; v < 5; ++v) wts[v] /= qdyn;
+  return PDES_OK;
+}
+
+// dJ/dq of the scalar functional `kind` (1..6) over the BC groups bcnums,
+// into the device vector d_out [numDof]
+int functional_grad_device(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* d_out) {
+  PDES_CHECK(c && d_out, PDES_ERR_ARG, "null argument");
+  const int nb = (int)(c->h_bndryfaces.size() / 2), Nf = c->Nf, dim = c->dim;
+  std::vector<int32_t> sel;
+  double wts[5];
+  int st0 = functional_setup(c, kind, bcnums, nbc, sel, wts);
+  if (st0) return st0;
   int32_t *d_bf = nullptr, *d_sel = nullptr;
   double *d_w = nullptr, *d_grad = nullptr;
   const size_t ngrad = (size_t)std::max(1, nb) * Nf * c->nd;
@@ -212,6 +271,57 @@ int pdes_functional_deriv(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc,
   if (st) return st;
   PDES_HIP(hipMemcpyAsync(dJdq, c->d_work, c->numDof * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
+  return PDES_OK;
+}
+
+// evalFunctionalDeriv_m: dJ/d(nrm_bndry) [numBoundaryFaces][Nf][dim] of a
+// scalar functional (kind 1..6); the functionals depend on no other metric
+int pdes_functional_deriv_m(pdes_ctx* c, int kind, const int32_t* bcnums, int nbc, double* nrm_bndry_bar) {
+  PDES_CHECK(c && nrm_bndry_bar, PDES_ERR_ARG, "null argument");
+  std::vector<int32_t> sel;
+  double wts[5];
+  int st = functional_setup(c, kind, bcnums, nbc, sel, wts);
+  if (st) return st;
+  const int nb = (int)(c->h_bndryfaces.size() / 2), Nf = c->Nf, dim = c->dim;
+  if (nb == 0) return PDES_OK;
+  int32_t *d_bf = nullptr, *d_sel = nullptr;
+  double *d_w = nullptr, *d_bar = nullptr;
+  const size_t nbar = (size_t)nb * Nf * dim;
+  hipError_t he = hipMalloc((void**)&d_bf, c->h_bndryfaces.size() * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc((void**)&d_sel, sel.size() * sizeof(int32_t));
+  if (he == hipSuccess) he = hipMalloc((void**)&d_w, sizeof(wts));
+  if (he == hipSuccess) he = hipMalloc((void**)&d_bar, nbar * sizeof(double));
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(d_bf, c->h_bndryfaces.data(), c->h_bndryfaces.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                        c->stream);
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(d_sel, sel.data(), sel.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream);
+  if (he == hipSuccess) he = hipMemcpyAsync(d_w, wts, sizeof(wts), hipMemcpyHostToDevice, c->stream);
+  bool launched = false;
+  if (he == hipSuccess) {
+    const int gb = (int)ceil_div((int64_t)nb * Nf, 128);
+#define NGRAD_LAUNCH(D, NFV, STV)                                                                           \
+    if (!launched && dim == D && Nf == NFV && c->stencil == STV) {                                         \
+      hipLaunchKernelGGL((functional_nrm_grad_kernel<D, NFV, STV>), dim3(gb), dim3(128), 0, c->stream,      \
+                         c->devmesh(), c->devsbp(), c->pc, c->d_q, c->Np, d_bf, nb, d_sel, d_w, d_bar);    \
+      launched = true;                                                                                      \
+    }
+    NGRAD_LAUNCH(2, 2, 3) NGRAD_LAUNCH(2, 3, 6) NGRAD_LAUNCH(2, 4, 10) NGRAD_LAUNCH(2, 5, 15)
+    NGRAD_LAUNCH(2, 3, 3) NGRAD_LAUNCH(2, 4, 4) NGRAD_LAUNCH(2, 5, 5) NGRAD_LAUNCH(2, 6, 6)
+    NGRAD_LAUNCH(3, 3, 4) NGRAD_LAUNCH(3, 6, 10)
+#undef NGRAD_LAUNCH
+    he = hipGetLastError();
+    if (launched && he == hipSuccess)
+      he = hipMemcpyAsync(nrm_bndry_bar, d_bar, nbar * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+  }
+  hipStreamSynchronize(c->stream);
+  hipFree(d_bf);
+  hipFree(d_sel);
+  hipFree(d_w);
+  hipFree(d_bar);
+  PDES_CHECK(he == hipSuccess, PDES_ERR_HIP, "functional metric derivative failed: %s", hipGetErrorString(he));
+  PDES_CHECK(launched, PDES_ERR_UNSUPPORTED, "no functional kernel for this operator");
   return PDES_OK;
 }
 

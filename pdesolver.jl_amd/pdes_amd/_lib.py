@@ -57,6 +57,8 @@ _SIGS = {
     "pdes_newton_solve": [_vp, C.POINTER(abi.NewtonOpts), C.POINTER(abi.NewtonReport)],
     "pdes_eval_functional": [_vp, C.c_int, _vp, C.c_int, _dp],
     "pdes_functional_deriv": [_vp, C.c_int, _vp, C.c_int, _vp],
+    "pdes_functional_deriv_m": [_vp, C.c_int, _vp, C.c_int, _vp],
+    "pdes_eval_residual_revm": [_vp, _vp, _vp, _vp, _vp, _vp],
     "pdes_bsr_matvec_transpose": [_vp, _vp, _vp],
     "pdes_eval_homotopy": [_vp, _vp],
     "pdes_set_homotopy_boundary": [_vp, C.c_int],

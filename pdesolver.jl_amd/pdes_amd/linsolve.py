@@ -126,6 +126,33 @@ def evalResidual_revq(mesh, sbp, eqn: EulerData, opts, input_array, output_array
         dst += out
 
 
+def evalResidual_revm(mesh, sbp, eqn: EulerData, opts, input_array, t: float = 0.0):
+    """evalResidual_revm (interface2.jl:329-372, evaldRdm.jl:19-70): adds
+    psi^T dR/dm, psi = input_array, into the mesh's `_bar` arrays
+    (dxidx_bar, nrm_face_bar, nrm_bndry_bar, coords_bndry_bar, created as
+    zeros on first use; zeroBarArrays resets them), at the device state."""
+    psi = np.ascontiguousarray(input_array, dtype=np.float64).reshape(-1)
+    assert psi.size == eqn.q.size
+    outs = {}
+    for name, ref in (("dxidx_bar", mesh.dxidx), ("nrm_face_bar", mesh.nrm_face),
+                      ("nrm_bndry_bar", mesh.nrm_bndry), ("coords_bndry_bar", mesh.coords_bndry)):
+        outs[name] = np.zeros(np.shape(ref), dtype=np.float64)
+    check(lib().pdes_eval_residual_revm(eqn.ctx, psi.ctypes.data, outs["dxidx_bar"].ctypes.data,
+                                        outs["nrm_face_bar"].ctypes.data, outs["nrm_bndry_bar"].ctypes.data,
+                                        outs["coords_bndry_bar"].ctypes.data))
+    for name, val in outs.items():
+        if getattr(mesh, name, None) is None:
+            setattr(mesh, name, np.zeros_like(val))
+        getattr(mesh, name)[...] += val
+
+
+def zeroBarArrays(mesh):
+    """zeroBarArrays: the mesh's `_bar` arrays set to zero."""
+    for name in ("dxidx_bar", "nrm_face_bar", "nrm_bndry_bar", "coords_bndry_bar"):
+        if getattr(mesh, name, None) is not None:
+            getattr(mesh, name)[...] = 0.0
+
+
 def setHomotopyBoundaryIntegrals(eqn: EulerData, on: bool):
     """opts["homotopy_addBoundaryIntegrals"] for this context's homotopy
     dissipation (default off)."""
@@ -170,6 +197,19 @@ def evalFunctionalDeriv_q(eqn: EulerData, functional: str, bcnums):
     out = np.empty(eqn.q.shape, dtype=np.float64)
     check(lib().pdes_functional_deriv(eqn.ctx, FUNCTIONALS[functional], bc.ctypes.data, len(bc), out.ctypes.data))
     return out
+
+
+def evalFunctionalDeriv_m(mesh, sbp, eqn: EulerData, opts, functional: str, bcnums, val_bar: float = 1.0):
+    """evalFunctionalDeriv_m (interface2.jl:205-227): adds val_bar dJ/dm into
+    the mesh's nrm_bndry_bar (the only metric the boundary functionals
+    depend on), at the device state."""
+    bc = _bcs(bcnums)
+    out = np.zeros(np.shape(mesh.nrm_bndry), dtype=np.float64)
+    check(lib().pdes_functional_deriv_m(eqn.ctx, FUNCTIONALS[functional], bc.ctypes.data, len(bc),
+                                        out.ctypes.data))
+    if getattr(mesh, "nrm_bndry_bar", None) is None:
+        mesh.nrm_bndry_bar = np.zeros_like(out)
+    mesh.nrm_bndry_bar += val_bar * out
 
 
 def calcAdjoint(mesh, sbp, eqn: EulerData, functional: str, bcnums, opts: "NewtonOptions | None" = None,

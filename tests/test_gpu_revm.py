@@ -1,0 +1,119 @@
+"""Reverse mode with respect to the mesh metrics on the device
+(evalResidual_revm, src/interface2.jl:329-372 -> src/solver/euler/evaldRdm.jl:
+19-70; evalFunctionalDeriv_m, interface2.jl:205-227 -> boundary_functional.jl:
+61-79) against the oracle: for random directions dm of each metric array
+(dxidx per node, nrm_face, nrm_bndry, coords_bndry), sum(m_bar * dm) must
+equal the derivative of psi . R_oracle(m + h dm) (or of J_oracle) at h = 0,
+taken by a fourth-order central difference of the oracle."""
+import copy
+
+import numpy as np
+import pytest
+
+from pdes_amd import abi, euler, linsolve, problems
+from pdes_amd._lib import PdesError
+from helpers import setup, state_for
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("dxidx", "nrm_face", "nrm_bndry", "coords_bndry")
+
+
+def _fd(fun, h):
+    return (-fun(2 * h) + 8 * fun(h) - 8 * fun(-h) + fun(-2 * h)) / (12 * h)
+
+
+def _perturbed(m, field, delta, h):
+    m2 = copy.copy(m)
+    setattr(m2, field, np.ascontiguousarray(getattr(m, field) + h * delta))
+    return m2
+
+
+CASES = [("vortex", 4, 2, "diage", "roe"), ("vortex", 4, 2, "diage", "ir"), ("vortex", 3, 4, "diage", "ir"),
+         ("vortex", 4, 2, "omega", "roe"), ("vortex", 3, 1, "omega", "roe"), ("vortex", 3, 3, "omega", "roe"),
+         ("cube", 2, 2, "omega", "roe"), ("cube", 2, 1, "omega", "roe")]
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", CASES)
+def test_residual_revm_matches_oracle_differences(oracle, kind, n, p, opk, flux):
+    print(f"case {kind} n={n} p={p} {opk} {flux}", flush=True)
+    m, op, ph, b = setup(kind, n, p, opk, flux)
+    q = state_for(kind, m, ph)
+    rng = np.random.default_rng(3)
+    psi = rng.standard_normal(q.shape)
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    linsolve.zeroBarArrays(m)
+    linsolve.evalResidual_revm(m, op, eqn, None, psi)
+    eqn.free()
+    for field in FIELDS:
+        bar = getattr(m, field + "_bar")
+        base = getattr(m, field)
+        assert bar.shape == base.shape
+        delta = rng.uniform(-1.0, 1.0, size=base.shape)
+        h = 1e-6 * max(1.0, np.abs(base).max())
+
+        def g(s):
+            m2 = _perturbed(m, field, delta, s)
+            return float(np.sum(psi * oracle.residual(abi.make_descriptors(m2, op, ph), q)))
+
+        ref = _fd(g, h)
+        got = float(np.sum(bar * delta))
+        scale = float(np.sum(np.abs(bar * delta))) + 1e-14
+        print(f"  {field}: device {got:.12e} oracle difference {ref:.12e} scale {scale:.3e}", flush=True)
+        assert abs(got - ref) <= 1e-7 * scale + 1e-11, field
+        if field == "coords_bndry" and kind == "cube":
+            assert np.all(bar == 0.0)      # freestream boundary states do not depend on the coordinates
+        elif kind == "vortex":
+            assert scale > 1e-8            # every metric array enters these residuals
+    # the reference adds into the _bar fields
+    before = m.dxidx_bar.copy()
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    linsolve.evalResidual_revm(m, op, eqn, None, psi)
+    eqn.free()
+    assert np.allclose(m.dxidx_bar, 2 * before, rtol=0, atol=1e-13 * np.abs(before).max())
+
+
+def test_residual_revm_rejects_entropy_stable_faces():
+    m, op, ph, b = setup("vortex", 3, 2, "omega", "es")
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(state_for("vortex", m, ph))
+    with pytest.raises(PdesError):
+        linsolve.evalResidual_revm(m, op, eqn, None, np.ones(eqn.q.shape))
+    eqn.free()
+
+
+@pytest.mark.parametrize("kind,n,p,opk,flux", [("vortex", 5, 2, "omega", "roe"), ("vortex", 4, 2, "diage", "ir"),
+                                              ("cube", 2, 2, "omega", "roe")])
+@pytest.mark.parametrize("functional", ["lift", "drag", "entropyflux", "massflow", "liftCoefficient"])
+def test_functional_deriv_m_matches_oracle_differences(oracle, kind, n, p, opk, flux, functional):
+    if functional == "massflow" and kind == "cube":
+        pytest.skip("MassFlowData's integrand is two-dimensional (boundary_functional.jl:404)")
+    print(f"case {kind} n={n} p={p} {opk} {flux} {functional}", flush=True)
+    key = {"lift": "lift", "drag": "drag", "entropyflux": "entropy", "massflow": "mass",
+           "liftCoefficient": "lift_coef"}[functional]
+    m, op, ph = problems.make_problem(kind, n, p, opk, flux, Ma=0.6, aoa_deg=3.0)
+    q = problems.vortex_noise_state(m, ph) if kind == "vortex" else problems.freestream_bump_state(m, ph)
+    bcs = list(range(m.numBC))[:2]
+    eqn = euler.EulerData(m, op, ph)
+    eqn.set_q(q)
+    linsolve.zeroBarArrays(m)
+    m.nrm_bndry_bar = np.zeros_like(m.nrm_bndry)
+    linsolve.evalFunctionalDeriv_m(m, op, eqn, None, functional, bcs)
+    eqn.free()
+    bar = m.nrm_bndry_bar
+    rng = np.random.default_rng(5)
+    delta = rng.uniform(-1.0, 1.0, size=m.nrm_bndry.shape)
+    h = 1e-6 * np.abs(m.nrm_bndry).max()
+    ref = _fd(lambda s: oracle.functional(_perturbed(m, "nrm_bndry", delta, s), op, ph, q, bcs)[key], h)
+    got = float(np.sum(bar * delta))
+    scale = float(np.sum(np.abs(bar * delta))) + 1e-14
+    print(f"  device {got:.12e} oracle difference {ref:.12e}", flush=True)
+    assert scale > 1e-10
+    assert abs(got - ref) <= 1e-7 * scale + 1e-11
+    # faces outside the selected BC groups get no derivative
+    off = np.ones(m.numBoundaryFaces, dtype=bool)
+    for g_ in bcs:
+        off[m.bndry_offsets[g_]:m.bndry_offsets[g_ + 1]] = False
+    assert np.all(bar[off] == 0.0)
