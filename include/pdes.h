@@ -341,9 +341,8 @@ int pdes_functional_deriv_m(pdes_ctx* ctx, int kind, const int32_t* bcnums, int 
  *   nrm_bndry_bar[numBoundaryFaces][Nf][dim],
  *   coords_bndry_bar[numBoundaryFaces][Nf][dim]  (through the boundary states).
  * The reference adds into the mesh's _bar fields, which its caller zeroes
- * first.  Any output may be NULL.  Weak form with Roe faces and split form
- * with IRSLF faces; PDES_ERR_UNSUPPORTED for the entropy-stable face integral
- * (face_flux ESLF) and for packed batches. */
+ * first.  Any output may be NULL.  Every volume form and face flux of the
+ * residual; PDES_ERR_UNSUPPORTED for packed batches. */
 int pdes_eval_residual_revm(pdes_ctx* ctx, const double* psi, double* dxidx_bar, double* nrm_face_bar,
                             double* nrm_bndry_bar, double* coords_bndry_bar);
 

@@ -3,7 +3,8 @@
 // for a seed psi (res_bar), the products psi^T dR/dm for
 //   dxidx        (volume terms: calcVolumeIntegrals_nopre_revm and
 //                 calcVolumeIntegralsSplitForm_revm, evaldRdm.jl:144-163),
-//   nrm_face     (interface fluxes: calcFaceIntegral_nopre_revm, :207-230),
+//   nrm_face     (interface fluxes: calcFaceIntegral_nopre_revm and
+//                 getFaceElementIntegral_revm, :207-230),
 //   nrm_bndry and coords_bndry (boundary fluxes and the boundary states:
 //                 getBCFluxes_revm, :176-184).
 //
@@ -138,7 +139,8 @@ __device__ __forceinline__ void isentropic_vortex_dual(const PhysConst& p, const
 
 // One thread per (element, face, face node).  Interior faces are taken from
 // their elementL side (each interface node once); the flux there is the one
-// the residual uses (Roe, or IRSLF with the split form), with weight
+// the residual uses (Roe, IRSLF, or the ESLF entropy penalty plus the
+// entropy-conservative face integral), with weight
 // w_k (psi_hat_R(kR) - psi_hat_L(k)): the face integral subtracts
 // R^T B F from elementL and adds it to elementR.  Boundary faces: Roe flux
 // against the boundary state, weight -w_k psi_hat_L(k).  Dual slots
@@ -210,17 +212,92 @@ __global__ void revm_face_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys, const
         pR[c] += r * psi[(nb * NP + node) * ND + c];
       }
     }
-    for (int c = 0; c < ND; ++c) qRd[c] = D(qR[c]);
-    if constexpr (FLUX == PDES_FLUX_ROE) roe_flux<DIM>(phys, qLd, qRd, nd, Fd);
-    else irslf_flux<DIM>(phys, qLd, qRd, nd, Fd);
-    for (int c = 0; c < ND; ++c) fbar[c] = w * (pR[c] - pL[c]);
+    if constexpr (FLUX == PDES_FLUX_ESLF) {
+      // ESLFFaceIntegral (faceElementIntegrals.jl:611-630): the Lax-Friedrichs
+      // entropy penalty at face node k (calcEntropyPenaltyIntegral :199-290)
+      // from the interpolated IR entropy variables of both sides, with the
+      // face weight inside the flux
+      double wl[ND], wr[ND];
+      for (int c = 0; c < ND; ++c) { wl[c] = 0.0; wr[c] = 0.0; }
+      for (int s = 0; s < ST; ++s) {
+        const int nodeL = sbp.perm[s * NFACES + f], nodeR = sbp.perm[s * NFACES + fN];
+        double qn[ND], wv[ND];
+        for (int c = 0; c < ND; ++c) qn[c] = q[(e * NP + nodeL) * ND + c];
+        to_ir_vars<DIM>(phys, qn, wv);
+        for (int c = 0; c < ND; ++c) wl[c] += sbp.interp[s * NF + k] * wv[c];
+        for (int c = 0; c < ND; ++c) qn[c] = q[(nb * NP + nodeR) * ND + c];
+        to_ir_vars<DIM>(phys, qn, wv);
+        for (int c = 0; c < ND; ++c) wr[c] += sbp.interp[s * NF + kR] * wv[c];
+      }
+      double qa[ND], qb[ND];
+      ir_to_conservative<DIM>(phys, wl, qa);
+      ir_to_conservative<DIM>(phys, wr, qb);
+      D qad[ND], dwd[ND];
+      for (int c = 0; c < ND; ++c) {
+        qad[c] = D(0.5 * (qa[c] + qb[c]));
+        dwd[c] = D(wl[c] - wr[c]);
+        Fd[c] = D(0.0);
+      }
+      lf_entropy_apply<DIM>(phys, qad, dwd, nd, w, Fd);
+      for (int c = 0; c < ND; ++c) fbar[c] = pR[c] - pL[c];
+    } else {
+      for (int c = 0; c < ND; ++c) qRd[c] = D(qR[c]);
+      if constexpr (FLUX == PDES_FLUX_ROE) roe_flux<DIM>(phys, qLd, qRd, nd, Fd);
+      else irslf_flux<DIM>(phys, qLd, qRd, nd, Fd);
+      for (int c = 0; c < ND; ++c) fbar[c] = w * (pR[c] - pL[c]);
+    }
   }
   double* out = (bnd ? nrm_bndry_bar : nrm_face_bar) + ((int64_t)rec.y * NF + k) * DIM;
+  double acc[DIM];
   for (int x = 0; x < DIM; ++x) {
-    double acc = 0.0;
-    for (int c = 0; c < ND; ++c) acc += fbar[c] * Fd[c].d[x];
-    out[x] = acc;
+    acc[x] = 0.0;
+    for (int c = 0; c < ND; ++c) acc[x] += fbar[c] * Fd[c].d[x];
   }
+  if constexpr (FLUX == PDES_FLUX_ESLF) {
+    if (!bnd) {
+      // the entropy-conservative part (calcECFaceIntegral :58-120): elementL's
+      // stencil node a against elementR's stencil node b, with
+      // E_ab = sum_k R[a][k] R[b][kR] w_k n[k], subtracts F_IR(q_a, q_b, E_ab)
+      // from node a and adds it to node b; F_IR is linear in E, so node k's
+      // normal gets R[a][k] R[b][kR] w_k (psi_b - psi_a) . F_IR^x(q_a, q_b)
+      const int64_t nb = rec.x;
+      const int fN = rec.z & 0xff;
+      const int o = (rec.z >> 8) & 0xff;
+      const int kR = sbp.nbrperm[k * sbp.norient + o];
+      double dir[DIM * DIM];
+      for (int x = 0; x < DIM; ++x)
+        for (int y = 0; y < DIM; ++y) dir[x * DIM + y] = x == y ? 1.0 : 0.0;
+      for (int a = 0; a < ST; ++a) {
+        const double ra = sbp.interp[a * NF + k];
+        if (ra == 0.0) continue;
+        const int nodeL = sbp.perm[a * NFACES + f];
+        double qa[ND], za[ND], pa[ND];
+        for (int c = 0; c < ND; ++c) {
+          qa[c] = q[(e * NP + nodeL) * ND + c];
+          pa[c] = psi[(e * NP + nodeL) * ND + c];
+        }
+        ir_zvars<DIM>(phys, qa, za);
+        for (int bb = 0; bb < ST; ++bb) {
+          const double coef = ra * sbp.interp[bb * NF + kR] * w;
+          if (coef == 0.0) continue;
+          const int nodeR = sbp.perm[bb * NFACES + fN];
+          double qb[ND], zb[ND], dp[ND], F[DIM * ND];
+          for (int c = 0; c < ND; ++c) {
+            qb[c] = q[(nb * NP + nodeR) * ND + c];
+            dp[c] = psi[(nb * NP + nodeR) * ND + c] - pa[c];
+          }
+          ir_zvars<DIM>(phys, qb, zb);
+          ir_flux_z<DIM, DIM>(phys, za, zb, dir, F);
+          for (int x = 0; x < DIM; ++x) {
+            double s2 = 0.0;
+            for (int c = 0; c < ND; ++c) s2 += dp[c] * F[x * ND + c];
+            acc[x] += coef * s2;
+          }
+        }
+      }
+    }
+  }
+  for (int x = 0; x < DIM; ++x) out[x] = acc[x];
   if (bnd) {
     double* oc = coords_bndry_bar + ((int64_t)rec.y * NF + k) * DIM;
     for (int x = 0; x < DIM; ++x) {
@@ -245,6 +322,10 @@ static int revm_launch(pdes_ctx* c, const double* psi, double* dxidx_bar, double
     hipLaunchKernelGGL((revm_face_kernel<DIM, NP, NF, ST, PDES_FLUX_ROE>), dim3((unsigned)ceil_div(nfn, 128)),
                        dim3(128), 0, c->stream, c->devmesh(), c->devsbp(), c->pc, c->d_q, psi, nrm_face_bar,
                        nrm_bndry_bar, coords_bndry_bar);
+  else if (c->phys.face_flux == PDES_FLUX_ESLF)
+    hipLaunchKernelGGL((revm_face_kernel<DIM, NP, NF, ST, PDES_FLUX_ESLF>), dim3((unsigned)ceil_div(nfn, 128)),
+                       dim3(128), 0, c->stream, c->devmesh(), c->devsbp(), c->pc, c->d_q, psi, nrm_face_bar,
+                       nrm_bndry_bar, coords_bndry_bar);
   else
     hipLaunchKernelGGL((revm_face_kernel<DIM, NP, NF, ST, PDES_FLUX_IRSLF>), dim3((unsigned)ceil_div(nfn, 128)),
                        dim3(128), 0, c->stream, c->devmesh(), c->devsbp(), c->pc, c->d_q, psi, nrm_face_bar,
@@ -263,8 +344,6 @@ int pdes_eval_residual_revm(pdes_ctx* c, const double* psi, double* dxidx_bar, d
                             double* nrm_bndry_bar, double* coords_bndry_bar) {
   PDES_CHECK(c && psi, PDES_ERR_ARG, "null argument");
   PDES_CHECK(c->nlist < 0 && c->ninst == 1, PDES_ERR_UNSUPPORTED, "pdes_eval_residual_revm takes a single-solve context");
-  PDES_CHECK(c->phys.face_flux == PDES_FLUX_ROE || c->phys.face_flux == PDES_FLUX_IRSLF, PDES_ERR_UNSUPPORTED,
-             "the metric reverse mode covers the Roe and IRSLF face fluxes (not the entropy-stable face integral)");
   const int dim = c->dim, Np = c->Np, Nf = c->Nf;
   const size_t n_dx = (size_t)c->numEl * Np * dim * dim, n_if = (size_t)std::max(1, c->numIface) * Nf * dim,
                n_bf = (size_t)std::max(1, c->numBndry) * Nf * dim;
