@@ -259,7 +259,8 @@ typedef struct {
   int32_t krylov_itermax;
   int32_t gmres_restart;       /* 30 */
   int32_t pc_kind;             /* PDES_PC_* */
-  int32_t fp64_blocks;         /* store Jacobian/PC in FP64 instead of FP32 */
+  int32_t fp64_blocks;         /* store the Jacobian blocks in FP64 instead of FP32; the preconditioner
+                                  factors (block inverses, ILU(0)) are FP32 either way */
   int32_t jv_mode;             /* 0 BSR, 1 matrix-free complex step */
   int32_t globalize_euler;     /* implicit-Euler globalization on/off */
   double euler_tau;            /* initial pseudo-time step (1.0) */

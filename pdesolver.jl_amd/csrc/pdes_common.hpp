@@ -193,15 +193,4 @@ template <class T> __device__ __forceinline__ T absvalue3(const T& val) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
-// f(std::integral_constant<int, 1>{}), ..., f(std::integral_constant<int, N>{})
-// in order: a loop whose counter is a compile-time constant in every trip
-template <class F, int... I>
-__device__ __forceinline__ void static_for_1_impl(F&& f, std::integer_sequence<int, I...>) {
-  (f(std::integral_constant<int, I + 1>{}), ...);
-}
-template <int N, class F>
-__device__ __forceinline__ void static_for_1(F&& f) {
-  static_for_1_impl(f, std::make_integer_sequence<int, N>{});
-}
-
 }  // namespace pdes
