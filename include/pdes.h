@@ -375,6 +375,10 @@ int pdes_batch_create(int ninst, const pdes_mesh_desc* meshes, const pdes_sbp_de
                       const pdes_phys_desc* phys, pdes_batch** out);
 int pdes_batch_destroy(pdes_batch* batch);
 int pdes_batch_num_groups(pdes_batch* batch, int* ngroups);
+/* Inspection (no counterpart in the reference): the Arnoldi blocks (8 GMRES
+ * steps between flag read-backs) captured into HIP graphs and the replays of
+ * those graphs so far, over all groups of the batch. */
+int pdes_batch_graph_stats(pdes_batch* batch, int64_t* captures, int64_t* replays);
 int pdes_batch_set_state(pdes_batch* batch, int inst, const double* q_host);
 int pdes_batch_get_state(pdes_batch* batch, int inst, double* q_host);
 int pdes_batch_newton_solve(pdes_batch* batch, const pdes_newton_opts* opts,

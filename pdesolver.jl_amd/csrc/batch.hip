@@ -25,6 +25,7 @@ int pc_check(pdes_ctx* c);
 int ilu_range_levels(const pdes_ctx* c, int64_t el0, int64_t el1, int* lf, int* lb);
 int ilu_hint(pdes_ctx* c, const int32_t* inst_sel, int ninst, int64_t nel, int lf, int lb);
 void ilu_hint_clear(pdes_ctx* c);
+void pc_graph_key(const pdes_ctx* c, int64_t key[6]);
 }  // namespace pdes
 
 namespace {
@@ -363,6 +364,14 @@ struct pdes_batch_group {
   double *d_H = nullptr, *d_cs = nullptr, *d_sn = nullptr, *d_g = nullptr, *d_gs = nullptr;
   int32_t* h_flags = nullptr;        // pinned readback of d_iter / d_live / d_total
   std::vector<int32_t> h_active;     // host copy of the ctx's per-instance work-list flags
+  // captured blocks of Arnoldi steps, replayed while their key matches
+  struct ArnoldiGraph {
+    std::vector<double> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<ArnoldiGraph> agraphs;
+  std::vector<std::vector<double>> akeys_seen;   // per block slot: the key of its last direct run
+  int64_t graph_replays = 0, graph_captures = 0;
   int gmres_R = 0;                   // restart length the H/cs/sn/g buffers are sized for
   // incremental solve (pdes_batch_advance): started after the first pass,
   // finished when no instance has work left
@@ -388,6 +397,8 @@ void group_free(pdes_batch_group* g) {
   if (g->h_flags) hipHostFree(g->h_flags);
   if (g->h_seg) hipHostFree(g->h_seg);
   if (g->h_coef) hipHostFree(g->h_coef);
+  for (auto& a : g->agraphs)
+    if (a.exec) hipGraphExecDestroy(a.exec);
   if (g->c) pdes_ctx_destroy(g->c);
   delete g;
 }
@@ -747,11 +758,14 @@ int group_pass(pdes_batch_group* g, const pdes_newton_opts* o) {
       }
       if (hlf > 0 && (st = ilu_hint(c, sel.data(), (int)sel.size(), hel, hlf, hlb))) return st;
     }
-    for (int j = 0; j < R; ++j) {
+    // one Arnoldi step: launches only (no host synchronization, no memset),
+    // so a block of steps can be captured into a graph
+    auto step = [&](int j) -> int {
       double* Vj = g->d_V + (int64_t)j * ldv;
       double* Vn = g->d_V + (int64_t)(j + 1) * ldv;
-      if ((st = pc_apply_masked(g, Vj, g->d_z, g->d_iter))) return st;
-      if ((st = apply_op_masked(g, o->jv_mode, g->d_z, Vn, g->d_iter))) return st;
+      int sst;
+      if ((sst = pc_apply_masked(g, Vj, g->d_z, g->d_iter))) return sst;
+      if ((sst = apply_op_masked(g, o->jv_mode, g->d_z, Vn, g->d_iter))) return sst;
       // CGS2: two passes of h = V^T w, w -= V h; H column = sum of both passes
       for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(seg_dot_partial, dim3(nch), dim3(SEG_THREADS), 0, sm, g->d_V, j + 1, ldv, Vn,
@@ -762,7 +776,7 @@ int group_pass(pdes_batch_group* g, const pdes_newton_opts* o) {
         hipLaunchKernelGGL(seg_multiaxpy, dim3(grid_n(n)), dim3(256), 0, sm, g->d_V, j + 1, ldv, g->d_coef, ms, Vn,
                            c->d_el_list, c->d_el_inst, g->dpe, n, g->d_iter);
       }
-      if ((st = seg_dots(g, Vn, 1, ldv, Vn, nullptr, nch, ni, g->d_iter))) return st;
+      if ((sst = seg_dots(g, Vn, 1, ldv, Vn, nullptr, nch, ni, g->d_iter))) return sst;
       hipLaunchKernelGGL(gmres_finish_step, dim3(gi), dim3(64), 0, sm, ni, g->d_ilist, g->d_iter, g->d_live,
                          g->d_total, g->d_jdone, g->d_seg, g->d_coef, ms, g->d_H, g->d_cs, g->d_sn, g->d_g, g->d_gs,
                          R, j, o->krylov_reltol, o->krylov_abstol, o->krylov_itermax);
@@ -770,7 +784,72 @@ int group_pass(pdes_batch_group* g, const pdes_newton_opts* o) {
       hipLaunchKernelGGL(seg_scale, dim3(grid_n(n)), dim3(256), 0, sm, Vn, Vn, g->d_coef, ms, 0, c->d_el_list,
                          c->d_el_inst, g->dpe, n, g->d_iter);
       PDES_HIP(hipGetLastError());
-      if (j + 1 < R && (j + 1) % 8 == 0) {
+      return PDES_OK;
+    };
+    // Blocks of 8 steps between flag read-backs.  A block is captured once
+    // into a HIP graph and replayed in later cycles while everything its
+    // launches depend on besides device data is unchanged: the steps, the
+    // work-list sizes (launch sizes), the operator and preconditioner
+    // configuration and the solver options (the work lists, masks and vectors
+    // live at fixed addresses).
+    for (int j0 = 0; j0 < R; j0 += 8) {
+      const int j1 = std::min(R, j0 + 8);
+      bool done_block = false;
+      if (c->use_graphs) {
+        int64_t pk[6];
+        pc_graph_key(c, pk);
+        std::vector<double> key = {(double)j0, (double)j1, (double)R, (double)c->nlist, (double)nch, (double)ni,
+                                   (double)o->jv_mode, (double)c->ptc_on, o->krylov_reltol, o->krylov_abstol,
+                                   (double)o->krylov_itermax};
+        for (int k = 0; k < 6; ++k) key.push_back((double)pk[k]);
+        hipGraphExec_t exec = nullptr;
+        for (auto& a : g->agraphs)
+          if (a.key == key) exec = a.exec;
+        // capture only a block whose key repeats from its last direct run:
+        // while the work lists change every cycle, capturing would cost more
+        // than the replays save
+        const size_t slot = (size_t)(j0 / 8);
+        if (g->akeys_seen.size() <= slot) g->akeys_seen.resize(slot + 1);
+        const bool repeat = g->akeys_seen[slot] == key;
+        if (!exec && !repeat) g->akeys_seen[slot] = key;
+        if (!exec && repeat) {
+          // the slot of these steps takes the new graph
+          for (auto it = g->agraphs.begin(); it != g->agraphs.end(); ++it)
+            if (it->key[0] == (double)j0) {
+              hipGraphExecDestroy(it->exec);
+              g->agraphs.erase(it);
+              break;
+            }
+          hipGraph_t graph = nullptr;
+          if (hipStreamBeginCapture(sm, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+            c->capturing = true;
+            int cst = PDES_OK;
+            for (int j = j0; j < j1 && !cst; ++j) cst = step(j);
+            c->capturing = false;
+            const hipError_t ec = hipStreamEndCapture(sm, &graph);
+            // a capture that failed runs the block directly (below), which
+            // reports the error if it was not the capture's
+            if (!cst && ec == hipSuccess && graph &&
+                hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+              g->agraphs.push_back({key, exec});
+              ++g->graph_captures;
+            }
+            else
+              exec = nullptr;
+            if (graph) hipGraphDestroy(graph);
+          }
+          (void)hipGetLastError();
+        }
+        if (exec) {
+          PDES_HIP(hipGraphLaunch(exec, sm));
+          ++g->graph_replays;
+          done_block = true;
+        }
+      }
+      if (!done_block)
+        for (int j = j0; j < j1; ++j)
+          if ((st = step(j))) return st;
+      if (j1 < R) {
         PDES_HIP(hipMemcpyAsync(g->h_flags, g->d_iter, g->inst.size() * 4, hipMemcpyDeviceToHost, sm));
         PDES_HIP(hipStreamSynchronize(sm));
         bool still = false;
@@ -1014,6 +1093,16 @@ int pdes_batch_destroy(pdes_batch* B) {
 int pdes_batch_num_groups(pdes_batch* B, int* ngroups) {
   PDES_CHECK(B && ngroups, PDES_ERR_ARG, "null argument");
   *ngroups = (int)B->groups.size();
+  return PDES_OK;
+}
+
+int pdes_batch_graph_stats(pdes_batch* B, int64_t* captures, int64_t* replays) {
+  PDES_CHECK(B && captures && replays, PDES_ERR_ARG, "null argument");
+  *captures = *replays = 0;
+  for (auto* g : B->groups) {
+    *captures += g->graph_captures;
+    *replays += g->graph_replays;
+  }
   return PDES_OK;
 }
 

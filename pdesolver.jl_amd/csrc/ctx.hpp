@@ -79,6 +79,9 @@ struct pdes_ctx {
   int64_t ilu_hint_el = -1;
   int ilu_hint_lf = 0, ilu_hint_lb = 0;
   bool ilu_sel = false;
+  // a caller is capturing this context's stream into a graph: launch
+  // everything directly (no nested graph replays)
+  bool capturing = false;
   int partial_n = 0;
   // residual launch variants (see residual_launch.hpp) for the plain and the
   // RK-epilogue kernel, chosen by the autotuner or the default table

@@ -1091,7 +1091,8 @@ int pc_apply(pdes_ctx* c, const double* b, double* x) {
   // the dependency-driven sweeps are 4 launches; the level-by-level ones are
   // replayed from a captured graph
   if (c->pc_obj && c->pc_obj->kind == PDES_PC_ILU0)
-    return c->use_graphs && !ilu_use_dep(c, c->pc_obj) ? ilu_apply_graphed(c, b, x) : ilu_apply_dispatch(c, b, x);
+    return c->use_graphs && !c->capturing && !ilu_use_dep(c, c->pc_obj) ? ilu_apply_graphed(c, b, x)
+                                                                        : ilu_apply_dispatch(c, b, x);
   if (!c->pc_obj || c->pc_obj->kind == PDES_PC_NONE) {
     const int64_t n = c->numDof;
     hipLaunchKernelGGL(copy_kernel, dim3((unsigned)std::min<int64_t>(8192, ceil_div(n, 256))), dim3(256), 0,
@@ -1118,6 +1119,26 @@ int pc_apply(pdes_ctx* c, const double* b, double* x) {
   }
   set_error("no block-Jacobi kernel for this operator");
   return PDES_ERR_UNSUPPORTED;
+}
+
+// What a captured sequence of preconditioner applications depends on besides
+// device data: the kind, the factor and matrix buffers, and for ILU(0) the
+// sweep kind and, for the dependency-driven sweeps over compacted chunk
+// lists, their lengths (launch sizes).
+void pc_graph_key(const pdes_ctx* c, int64_t key[6]) {
+  const pdes_precond* p = c->pc_obj;
+  for (int i = 0; i < 6; ++i) key[i] = 0;
+  if (!p) return;
+  key[0] = p->kind;
+  key[1] = (int64_t)(intptr_t)p->d_inv;
+  key[2] = c->jac ? (int64_t)(intptr_t)c->jac->d_vals : 0;
+  key[3] = c->jac ? (int64_t)c->jac->fp64 : 0;
+  if (p->kind == PDES_PC_ILU0) {
+    const bool dep = ilu_use_dep(c, p);
+    const bool sel = dep && c->ilu_sel && c->ilu_hint_el >= 0;
+    key[4] = dep ? 1 + (sel ? 1 : 0) : 0;
+    key[5] = sel ? p->sel_nf * (int64_t(1) << 32) + p->sel_nb : 0;
+  }
 }
 
 // the error word of the dependency-driven sweeps (a wait that gave up): the

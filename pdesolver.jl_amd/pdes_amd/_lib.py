@@ -70,6 +70,7 @@ _SIGS = {
     "pdes_batch_create": [C.c_int, _vp, _vp, _vp, C.POINTER(_vp)],
     "pdes_batch_destroy": [_vp],
     "pdes_batch_num_groups": [_vp, C.POINTER(C.c_int)],
+    "pdes_batch_graph_stats": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
     "pdes_batch_set_state": [_vp, C.c_int, _vp],
     "pdes_batch_get_state": [_vp, C.c_int, _vp],
     "pdes_batch_newton_solve": [_vp, C.POINTER(abi.NewtonOpts), _vp],

@@ -38,6 +38,12 @@ class BatchSolver:
         check(lib().pdes_batch_num_groups(self.handle, C.byref(g)))
         return g.value
 
+    def graph_stats(self):
+        """(Arnoldi blocks captured into HIP graphs, graph replays) so far."""
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().pdes_batch_graph_stats(self.handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def shape(self, i):
         m, op, _ = self.problems[i]
         return (m.numEl, op.numnodes, m.numDofPerNode)

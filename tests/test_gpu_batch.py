@@ -185,3 +185,42 @@ def test_batch_ilu_sweep_kinds_bit_identical(monkeypatch):
     assert all(r[0] == 1 for r in rd)
     for i in range(len(probs)):
         assert np.array_equal(qd[i], ql[i]), f"instance {i} differs between the sweep kinds"
+
+
+@pytest.mark.parametrize("pc_kind,globalize,forced", [(2, False, "PDES_ILU_DEP"), (2, False, "PDES_ILU_LEVELS"),
+                                                      (1, False, None), (2, True, None)])
+def test_batch_graph_replays_match_direct_launches(monkeypatch, pc_kind, globalize, forced):
+    """The batched GMRES captures each block of 8 Arnoldi steps into a HIP
+    graph once its launch configuration repeats, and replays it in later
+    cycles.  Replayed and directly launched blocks (a batch created with
+    PDES_NO_GRAPHS) give the same bits, Newton and Krylov counts, with
+    block-Jacobi and both ILU(0) sweep kinds, with and without the
+    pseudo-transient term; the graphed run did replay graphs."""
+    opts = linsolve.NewtonOptions(res_abstol=1e-10, krylov_reltol=1e-5, itermax=30, pc_kind=pc_kind,
+                                  globalize_euler=globalize, euler_tau=1.0)
+    probs = []
+    for i, (n, Min) in enumerate([(14, 0.5), (9, 0.7), (12, 0.4)]):
+        m, op, ph = problems.make_problem("vortex", n, 2, "diage", "ir", vortex_Min=Min)
+        probs.append((m, op, ph, problems.vortex_noise_state(m, ph, seed=30 + i)))
+    if forced:
+        monkeypatch.setenv(forced, "1")
+    out = {}
+    for graphs in (True, False):
+        if graphs:
+            monkeypatch.delenv("PDES_NO_GRAPHS", raising=False)
+        else:
+            monkeypatch.setenv("PDES_NO_GRAPHS", "1")
+        bs = BatchSolver([(m, op, ph) for (m, op, ph, q) in probs])
+        for i, p in enumerate(probs):
+            bs.set_state(i, p[3])
+        reps = bs.solve(opts)
+        out[graphs] = ([bs.get_state(i) for i in range(len(probs))],
+                       [(r.converged, r.iterations, r.total_krylov_iters) for r in reps], bs.graph_stats())
+        bs.free()
+        print("graphs" if graphs else "direct", out[graphs][1], "captures/replays", out[graphs][2], flush=True)
+    (qg, rg, sg), (qd, rd, sd) = out[True], out[False]
+    assert rg == rd
+    assert all(r[0] == 1 for r in rg)
+    assert sg[1] > 0 and sd == (0, 0)
+    for i in range(len(probs)):
+        assert np.array_equal(qg[i], qd[i]), f"instance {i} differs between graph replays and direct launches"
