@@ -495,7 +495,8 @@ ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, con
 // sets the error word and carries on, so every launch ends.
 // ---------------------------------------------------------------------------
 constexpr int ILU_WAIT_POLLS = 1 << 18;
-enum { ILU_SYNC_TICKET = 0, ILU_SYNC_EPOCH = 1, ILU_SYNC_ERR = 2, ILU_SYNC_WORDS = 4 };
+// sync words: ticket counter, sweep epoch, error word, poll limit (0: ILU_WAIT_POLLS)
+enum { ILU_SYNC_TICKET = 0, ILU_SYNC_EPOCH = 1, ILU_SYNC_ERR = 2, ILU_SYNC_POLLS = 3, ILU_SYNC_WORDS = 4 };
 
 // elements per chunk for a row block of n unknowns: as many element row
 // blocks as fill a workgroup of at most 256 threads, at most 4
@@ -545,6 +546,7 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
   const int tt = t - slot * TPE;
   const int r = tt / LPR, l = tt - r * LPR;
   const int32_t epoch = sync[ILU_SYNC_EPOCH];
+  const int32_t max_polls = sync[ILU_SYNC_POLLS] > 0 ? sync[ILU_SYNC_POLLS] : ILU_WAIT_POLLS;
   if (t == 0) {
     const int64_t tk = __hip_atomic_fetch_add(&sync[ILU_SYNC_TICKET], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int64_t ch = tk < nchunks ? (chunks ? (int64_t)chunks[tk] : tk) : -1;
@@ -600,7 +602,7 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
       if (!(forward ? (k >= 0 && k < e) : (k > e))) continue;
       int polls = 0;
       while (__hip_atomic_load(&done[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++polls > ILU_WAIT_POLLS) {
+        if (++polls > max_polls) {
           __hip_atomic_store(&sync[ILU_SYNC_ERR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
@@ -759,6 +761,14 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
   }
   PDES_HIP(hipMemsetAsync(p->d_done, 0, std::max(1, nel) * sizeof(int32_t), c->stream));
   PDES_HIP(hipMemsetAsync(p->d_sync, 0, ILU_SYNC_WORDS * sizeof(int32_t), c->stream));
+  // PDES_ILU_WAIT_POLLS: a smaller bound on the dependency waits (for tests
+  // of the give-up path; the default bound is never reached by a sweep that
+  // runs normally)
+  const char* polls_env = std::getenv("PDES_ILU_WAIT_POLLS");
+  const int32_t poll_limit = polls_env ? (int32_t)std::max(1L, std::strtol(polls_env, nullptr, 10)) : 0;
+  if (poll_limit)
+    PDES_HIP(hipMemcpyAsync(p->d_sync + ILU_SYNC_POLLS, &poll_limit, sizeof(int32_t), hipMemcpyHostToDevice,
+                            c->stream));
   PDES_HIP(hipStreamSynchronize(c->stream));
   p->ilu_ready = true;
   return PDES_OK;

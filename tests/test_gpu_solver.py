@@ -495,3 +495,35 @@ def test_entropy_stable_omega_jacobian_and_newton(oracle, kind, n, p):
     print("ES Newton iterations", rep.iterations, "krylov", rep.total_krylov_iters, "norm", rep.res_norm)
     assert rep.converged and rep.res_norm <= 1e-10
     eqn.free()
+
+
+def test_ilu0_dependency_wait_limit_reports_an_error(monkeypatch):
+    """A dependency-driven ILU(0) sweep whose waits give up (forced here by a
+    one-poll bound, PDES_ILU_WAIT_POLLS=1) still ends every launch, and the
+    solver reports the invalid preconditioner application as an error
+    instead of returning its result; the same solve with the normal bound
+    succeeds."""
+    from pdes_amd._lib import PdesError
+    m, op, ph = problems.make_problem("vortex", 40, 2, "diage", "ir")
+    q = problems.vortex_noise_state(m, ph, seed=5)
+    monkeypatch.setenv("PDES_ILU_DEP", "1")
+    for limited in (True, False):
+        if limited:
+            monkeypatch.setenv("PDES_ILU_WAIT_POLLS", "1")
+        else:
+            monkeypatch.delenv("PDES_ILU_WAIT_POLLS", raising=False)
+        e = euler.EulerData(m, op, ph)
+        e.set_q(q)
+        linsolve.evalJacobian(m, op, e)
+        pc = linsolve.HIPILU0PC(e)
+        pc.calcPC()
+        b = np.random.default_rng(2).standard_normal(q.shape)
+        x = np.zeros_like(b)
+        if limited:
+            with pytest.raises(PdesError, match="gave up waiting"):
+                for _ in range(5):
+                    pc.applyPC(m, op, e, None, 0.0, b, x)
+        else:
+            pc.applyPC(m, op, e, None, 0.0, b, x)
+            assert np.all(np.isfinite(x))
+        e.free()
