@@ -167,7 +167,12 @@ struct ResidualSmem {
   int32_t perm[Op::ST][NFACES];
   int32_t nodepos[NP][NFACES];
   double wface[NF];
-  T q[EPB][NP][ND];
+  // the element states, for the face phase of dense-face operators (which
+  // interpolates over all stencil nodes); diagonal-E face slots read their
+  // one own node from global memory instead (an L2 hit: the element was just
+  // read), which leaves room for one more workgroup per CU
+  static constexpr bool Q_LDS = !Op::DIAGE;
+  T q[Q_LDS ? EPB : 1][Q_LDS ? NP : 1][ND];
   // weak form: volume fluxes; split form: IR z-variables and partner contributions
   // (the F and C arrays are planes of one value per (element, node),
   //  [component][element][node], so that the lanes of a pair round, which
@@ -318,7 +323,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
     for (int k = 0; k < ND; ++k) {
       qi[k] = Io<T>::load(qin, vin, (e * NP + i) * ND + k);
-      sm.q[el][i][k] = qi[k];
+      if constexpr (Smem::Q_LDS) sm.q[el][i][k] = qi[k];
     }
     // checkDensity / checkPressure (euler.jl:543-611)
     if (flag) {
@@ -496,7 +501,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           sign = -1.0;
 #pragma unroll
           for (int c = 0; c < ND; ++c) {
-            qL[c] = sm.q[el][node][c];
+            qL[c] = Io<T>::load(qin, vin, (e * NP + node) * ND + c);
             qR[c] = qn[c];
           }
           roe_flux<DIM>(phys, qL, qR, nrm, Fk);
@@ -504,7 +509,7 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
           sign = side == 0 ? -1.0 : 1.0;
 #pragma unroll
           for (int c = 0; c < ND; ++c) {
-            const T qo = sm.q[el][node][c];
+            const T qo = Io<T>::load(qin, vin, (e * NP + node) * ND + c);
             qL[c] = side == 0 ? qo : qn[c];
             qR[c] = side == 0 ? qn[c] : qo;
           }
@@ -805,9 +810,12 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
 #pragma unroll
       for (int c = 0; c < ND; ++c) {
         const double k = re(ri[c]) * minv;
-        // stage 0: q_old is this stage's input, still staged in LDS (reading
-        // it there lets the entry registers qi die after the volume phase)
-        const double qo = rk.stage == 0 ? re(sm.q[el][i][c]) : rk.q_old[base + c];
+        // stage 0: q_old is this stage's input, still staged in LDS where
+        // the operator stages it (reading it there lets the entry registers
+        // qi die after the volume phase)
+        double qo;
+        if constexpr (Smem::Q_LDS) qo = rk.stage == 0 ? re(sm.q[el][i][c]) : rk.q_old[base + c];
+        else qo = rk.q_old[base + c];
         if (rk.stage == 3) {
           rk.q_next[base + c] = qo + rk.h6 * (rk.acc[base + c] + rk.wacc * k);
         } else {
