@@ -1202,7 +1202,7 @@ int pc_setup(pdes_ctx* c, int kind) {
 // GMRES building blocks
 // ---------------------------------------------------------------------------
 constexpr int MD_THREADS = 256;
-constexpr int MD_BLOCKS = 512;
+constexpr int MD_BLOCKS = 2048;
 
 // partial[i][b] = sum over block b's slice of V_i . w.  MD_GROUP basis
 // vectors per pass over the slice, so w is read once per group; each vector's
