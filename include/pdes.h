@@ -144,8 +144,9 @@ int pdes_eval_residual(pdes_ctx* ctx, double t);
  * launch variants 0-3 (elements per workgroup 256/Np or 128/Np x 3 or 4 waves
  * per SIMD) and, for the weak form, 4-7: the same shapes with the volume term
  * Q^T F on the FP64 matrix cores (the split form runs variant v & 3 for
- * these); pdes_autotune times each on this context's mesh, for the plain and
- * the RK4-stage form, and keeps the fastest (chosen[0], chosen[1]).
+ * these); pdes_autotune times each on this context's mesh (iters launches,
+ * best of three interleaved rounds), for the plain and the RK4-stage form, and
+ * keeps the fastest (chosen[0], chosen[1]).
  * pdes_set_launch_variant (0..7) fixes them. */
 int pdes_autotune(pdes_ctx* ctx, int iters, int32_t* chosen);
 int pdes_set_launch_variant(pdes_ctx* ctx, int plain_variant, int rk_variant);

@@ -1,5 +1,8 @@
 // residual.hip -- operator-shape dispatch of the fused residual kernel and the
 // reduction / vector kernels used around it.
+#include <algorithm>
+#include <vector>
+
 #include "residual_launch.hpp"
 #include "shapes.hpp"
 
@@ -122,27 +125,34 @@ int pdes_autotune(pdes_ctx* c, int iters, int32_t* chosen) {
   hipEvent_t a, b;
   PDES_HIP(hipEventCreate(&a));
   PDES_HIP(hipEventCreate(&b));
+  // every variant is timed in each of AUTOTUNE_ROUNDS interleaved rounds and
+  // keeps its best round, so a clock ramp at the start or a one-off stall
+  // does not decide the choice
+  constexpr int AUTOTUNE_ROUNDS = 3;
+  const int nvar = res_variant_count(c);
   for (int rkmode = 0; rkmode < 2; ++rkmode) {
-    float best = 1e30f;
-    int bestv = c->res_variant[rkmode];
-    for (int var = 0; var < res_variant_count(c); ++var) {
-      c->res_variant[rkmode] = var;
-      // stage-1 epilogue: reads q and the accumulator, writes scratch buffers only
-      RKStage rk{c->d_Minv_node, c->d_q, c->d_work, c->d_work2, 2.0, 0.0, 0.0, 1};
-      const RKStage* rkp = rkmode ? &rk : nullptr;
-      int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);  // warm-up
-      if (st) return st;
-      PDES_HIP(hipEventRecord(a, c->stream));
-      for (int it = 0; it < iters; ++it) {
-        st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);
+    std::vector<float> best(nvar, 1e30f);
+    for (int round = 0; round < AUTOTUNE_ROUNDS; ++round) {
+      for (int var = 0; var < nvar; ++var) {
+        c->res_variant[rkmode] = var;
+        // stage-1 epilogue: reads q and the accumulator, writes scratch buffers only
+        RKStage rk{c->d_Minv_node, c->d_q, c->d_work, c->d_work2, 2.0, 0.0, 0.0, 1};
+        const RKStage* rkp = rkmode ? &rk : nullptr;
+        int st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);  // warm-up
         if (st) return st;
+        PDES_HIP(hipEventRecord(a, c->stream));
+        for (int it = 0; it < iters; ++it) {
+          st = launch_residual(c, c->d_q, nullptr, c->d_res, nullptr, false, rkp);
+          if (st) return st;
+        }
+        PDES_HIP(hipEventRecord(b, c->stream));
+        PDES_HIP(hipEventSynchronize(b));
+        float ms = 0;
+        PDES_HIP(hipEventElapsedTime(&ms, a, b));
+        best[var] = std::min(best[var], ms);
       }
-      PDES_HIP(hipEventRecord(b, c->stream));
-      PDES_HIP(hipEventSynchronize(b));
-      float ms = 0;
-      PDES_HIP(hipEventElapsedTime(&ms, a, b));
-      if (ms < best) { best = ms; bestv = var; }
     }
+    const int bestv = (int)(std::min_element(best.begin(), best.end()) - best.begin());
     c->res_variant[rkmode] = bestv;
     if (chosen) chosen[rkmode] = bestv;
   }
