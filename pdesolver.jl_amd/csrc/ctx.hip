@@ -322,9 +322,16 @@ int pdes_estimate_bytes(const pdes_mesh_desc* m, const pdes_sbp_desc* s, int fp6
   b += (nel * Np + 1) * 8 + nnzb * 4 + nnzb * nd * nd * (fp64_blocks ? 8 : 4) + 4;
   // preconditioner: one FP32 inverse per element (block-Jacobi and ILU(0))
   if (pc_kind != PDES_PC_NONE) b += nel * (Np * nd) * (Np * nd) * 4;
-  if (pc_kind == PDES_PC_ILU0) b += 2 * nel * 4 + 2 * ndof * 8;   // level lists, graph staging
-  // GMRES basis and vectors, Newton step and right-hand side
-  b += (gmres_restart + 1) * ndof * 8 + 2 * ndof * 8 + (int64_t)(gmres_restart + 2) * 512 * 8;
+  if (pc_kind == PDES_PC_ILU0) {
+    b += 2 * nel * 4 + 2 * ndof * 8;   // level lists, graph staging
+    // own-block offsets per node row, the padded chunk lists (a level padded
+    // to whole chunks of at most 4 elements: at most 4 entries per element)
+    // and the dependency-driven sweeps' vector
+    b += nel * Np * 4 + 2 * 4 * nel * 4 + ndof * 8;
+  }
+  // GMRES basis and vectors (multi-dot partials: 2048 workgroups per basis
+  // vector), Newton step and right-hand side
+  b += (gmres_restart + 1) * ndof * 8 + 2 * ndof * 8 + (int64_t)(gmres_restart + 2) * 2048 * 8;
   b += 2 * ndof * 8;
   if (batched) b += 2 * nel * 4 + 2 * ndof * 8 + (ndof / 4096 + 1) * (8 + 4 + 4 + 64 * 8);
   *bytes = b;

@@ -33,9 +33,10 @@ struct pdes_precond {
   int32_t* d_fel = nullptr;
   int32_t* d_bel = nullptr;
   int32_t* d_own = nullptr;   // per node row: offset of the element's own block run within the row
-  // dependency-driven sweeps: per-element completion flags (the epoch of the
-  // sweep that last finished the element) and the ticket / epoch / error words
-  int32_t* d_done = nullptr;
+  // dependency-driven sweeps: the forward sweep's result (a vector of the
+  // unknowns that holds ILU_PENDING between applications) and the ticket /
+  // error / poll-limit words
+  double* d_y = nullptr;
   int32_t* d_sync = nullptr;
   // sweep choice: 0 by the cost model, 1 always level by level
   // (PDES_ILU_LEVELS set), 2 always dependency-driven (PDES_ILU_DEP set)
@@ -84,7 +85,7 @@ int precond_free(pdes_ctx* c) {
   if (c->pc_obj->d_fel) hipFree(c->pc_obj->d_fel);
   if (c->pc_obj->d_bel) hipFree(c->pc_obj->d_bel);
   if (c->pc_obj->d_own) hipFree(c->pc_obj->d_own);
-  if (c->pc_obj->d_done) hipFree(c->pc_obj->d_done);
+  if (c->pc_obj->d_y) hipFree(c->pc_obj->d_y);
   if (c->pc_obj->d_fpad) hipFree(c->pc_obj->d_fpad);
   if (c->pc_obj->d_bpad) hipFree(c->pc_obj->d_bpad);
   if (c->pc_obj->d_fsel) hipFree(c->pc_obj->d_fsel);
@@ -483,20 +484,24 @@ ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, con
 // chunk, in ticket order: an atomic counter hands out chunk positions, so
 // every element that a chunk's elements depend on (their lower neighbours in
 // the forward sweep, their upper ones in the backward sweep) sits in a chunk
-// with a smaller ticket, which already runs on the device or is done.  The
-// first thread of each element waits for those neighbours' completion flags
-// to carry this sweep's epoch, the block computes x_e as the level kernels do
-// (same lanes, same order of operations: the same bits), stores it, waits for
-// its stores to complete and then sets e's flag.  The neighbours' x are read
-// and the own x written with device-scope atomic loads/stores, so no stale
-// copy in another XCD's L2 is ever read.  The flags are never cleared: the
-// epoch, advanced by ilu_sweep_begin before each sweep, tells this sweep's
-// flags from older ones.  A wait gives up after a bounded number of polls,
-// sets the error word and carries on, so every launch ends.
+// with a smaller ticket, which already runs on the device or is done.  A
+// sweep's output holds ILU_PENDING (a signalling-NaN bit pattern, which no
+// arithmetic produces: a NaN result is quiet) wherever it is not yet
+// computed, so a workgroup waits for its neighbours by loading their values
+// until none is pending, then computes x_e as the level kernels do (same
+// lanes, same order of operations: the same bits) and stores it.  Values are
+// loaded and stored with device-scope atomics (an 8-byte store is seen whole
+// or not at all), so no stale copy in another XCD's L2 is ever read.  The
+// forward sweep writes y = L^-1 b into d_y and sets its elements' rows of x
+// pending (after reading b: x may be b); the backward sweep reads its own y,
+// writes x and sets its rows of y pending again, so d_y is pending between
+// applications (it starts so).  A wait gives up after a bounded number of
+// polls, sets the error word and carries on, so every launch ends.
 // ---------------------------------------------------------------------------
 constexpr int ILU_WAIT_POLLS = 1 << 18;
-// sync words: ticket counter, sweep epoch, error word, poll limit (0: ILU_WAIT_POLLS)
-enum { ILU_SYNC_TICKET = 0, ILU_SYNC_EPOCH = 1, ILU_SYNC_ERR = 2, ILU_SYNC_POLLS = 3, ILU_SYNC_WORDS = 4 };
+constexpr unsigned long long ILU_PENDING = 0x7FF4DEADBEEF0001ull;
+// sync words: ticket counter, error word, poll limit (0: ILU_WAIT_POLLS)
+enum { ILU_SYNC_TICKET = 0, ILU_SYNC_ERR = 1, ILU_SYNC_POLLS = 2, ILU_SYNC_WORDS = 3 };
 
 // elements per chunk for a row block of n unknowns: as many element row
 // blocks as fill a workgroup of at most 256 threads, at most 4
@@ -507,9 +512,15 @@ __host__ __device__ constexpr int ilu_dep_chunk(int n, bool diage) {
   return 256 / (n * ilu_dep_lanes(n, diage)) >= 4 ? 4 : 256 / (n * ilu_dep_lanes(n, diage));
 }
 
-__global__ void ilu_sweep_begin(int32_t* __restrict__ sync) {
-  sync[ILU_SYNC_TICKET] = 0;
-  sync[ILU_SYNC_EPOCH] = sync[ILU_SYNC_EPOCH] + 1;
+__global__ void ilu_sweep_begin(int32_t* __restrict__ sync) { sync[ILU_SYNC_TICKET] = 0; }
+
+__global__ void ilu_pending_fill(double* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __longlong_as_double((long long)ILU_PENDING);
+}
+
+__device__ __forceinline__ bool ilu_is_pending(double v) {
+  return (unsigned long long)__double_as_longlong(v) == ILU_PENDING;
 }
 
 __device__ __forceinline__ double x_load(const double* p) {
@@ -519,21 +530,24 @@ __device__ __forceinline__ void x_store(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// in: the right-hand side (forward: b; backward: y); out: the result, which
+// the neighbours' values are read from (forward: y; backward: x); reset: the
+// vector whose rows of this element become pending after use (forward: x;
+// backward: y).  in may alias reset.
 template <int NP, int ND, int NS, bool DIAGE, class BT, int LPR, int CH>
 __global__ void __launch_bounds__(256)
 ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restrict__ chunks, int64_t nchunks,
                      const int64_t* __restrict__ rowptr, const int32_t* __restrict__ colind,
                      const int32_t* __restrict__ own, const BT* __restrict__ vals, const float* __restrict__ inv,
-                     const double* __restrict__ b, double* __restrict__ x, int forward,
+                     const double* in, double* out, double* reset, int forward,
                      const int32_t* __restrict__ el_inst, const int32_t* __restrict__ mask,
-                     const int32_t* __restrict__ inst_active, const pdes::ElFace* __restrict__ elface, int nfaces,
-                     int32_t* __restrict__ done, int32_t* __restrict__ sync) {
+                     const int32_t* __restrict__ inst_active, int32_t* __restrict__ sync) {
   constexpr int N = NP * ND;
   constexpr int TPE = N * LPR;
   static_assert(CH * TPE <= 256, "a chunk's rows must fit a workgroup");
   // neighbour blocks of one row on one side: at most one per face with
   // diagonal-E faces, the face stencil per face otherwise; this lane's share
-  // is loaded while the waits run
+  // is loaded before the neighbours' values are awaited
   constexpr int NFACES = (ND == 4) ? 3 : 4;
   constexpr int MAXB = DIAGE ? NFACES : (NFACES * NS + LPR - 1) / LPR;
   __shared__ double tp[CH][N][LPR];
@@ -545,7 +559,6 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
   const int slot = t / TPE;
   const int tt = t - slot * TPE;
   const int r = tt / LPR, l = tt - r * LPR;
-  const int32_t epoch = sync[ILU_SYNC_EPOCH];
   const int32_t max_polls = sync[ILU_SYNC_POLLS] > 0 ? sync[ILU_SYNC_POLLS] : ILU_WAIT_POLLS;
   if (t == 0) {
     const int64_t tk = __hip_atomic_fetch_add(&sync[ILU_SYNC_TICKET], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -566,10 +579,10 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
   if (!sh_any) return;   // uniform over the block
   const int64_t e = slot < CH ? sh_e[slot] : -1;
   const bool act = e >= 0;
-  // everything that does not depend on the neighbours' x: this lane's
+  // everything that does not depend on the neighbours' values: this lane's
   // entries of D'_e^-1 (columns l, l + LPR, ...; the first 32 when there are
-  // more than 72), the right-hand side (forward) or the own x (backward),
-  // and this lane's neighbour blocks and their columns
+  // more than 72), the right-hand side, and this lane's neighbour blocks and
+  // their columns
   constexpr int NC = (N + LPR - 1) / LPR;
   constexpr int PF = NC <= 72 ? NC : 32;
   float mrow[PF];
@@ -582,7 +595,7 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
 #pragma unroll
     for (int k = 0; k < PF; ++k)
       if (l + k * LPR < N) mrow[k] = Mi[(l + k * LPR) * N + r];
-    if (l == 0) bval = forward ? b[e * N + r] : x[e * N + r];
+    if (l == 0) bval = in[e * N + r];
     const int i = r / ND, a = r - i * ND;
     const int64_t row = e * NP + i;
     const int64_t p0 = rowptr[row], p1 = rowptr[row + 1];
@@ -596,38 +609,60 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
       for (int c = 0; c < ND; ++c) pre[u][c] = p < q1 ? vals[p * ND * ND + a * ND + c] : BT(0);
     }
   }
-  if (act && tt == 0) {
-    for (int f = 0; f < nfaces; ++f) {
-      const int64_t k = elface[e * nfaces + f].x;
-      if (!(forward ? (k >= 0 && k < e) : (k > e))) continue;
-      int polls = 0;
-      while (__hip_atomic_load(&done[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (++polls > max_polls) {
-          __hip_atomic_store(&sync[ILU_SYNC_ERR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-  }
-  __syncthreads();
   if (act) {
     const int a = r % ND;
+    // the neighbours' values: all loads issued at once, then the pending
+    // ones polled again until every one has arrived
+    double xv[MAXB][ND];
+#pragma unroll
+    for (int u = 0; u < MAXB; ++u)
+#pragma unroll
+      for (int c = 0; c < ND; ++c)
+        xv[u][c] = q0 + l + (int64_t)u * LPR < q1 ? x_load(out + (int64_t)pcol[u] * ND + c) : 0.0;
+    int polls = 0;
+    while (true) {
+      bool pending = false;
+#pragma unroll
+      for (int u = 0; u < MAXB; ++u)
+#pragma unroll
+        for (int c = 0; c < ND; ++c) pending |= ilu_is_pending(xv[u][c]);
+      if (!pending) break;
+      if (++polls > max_polls) {
+        __hip_atomic_store(&sync[ILU_SYNC_ERR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int u = 0; u < MAXB; ++u)
+#pragma unroll
+        for (int c = 0; c < ND; ++c)
+          if (ilu_is_pending(xv[u][c])) xv[u][c] = x_load(out + (int64_t)pcol[u] * ND + c);
+    }
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
       if (q0 + l + (int64_t)u * LPR < q1) {
-        const double* xc = x + (int64_t)pcol[u] * ND;
 #pragma unroll
-        for (int c = 0; c < ND; ++c) acc += (double)pre[u][c] * x_load(xc + c);
+        for (int c = 0; c < ND; ++c) acc += (double)pre[u][c] * xv[u][c];
       }
     }
     for (int64_t p = q0 + l + (int64_t)MAXB * LPR; p < q1; p += LPR) {
       const int32_t col = colind[p];
       const BT* blk = vals + p * ND * ND + a * ND;
-      const double* xc = x + (int64_t)col * ND;
+      const double* xc = out + (int64_t)col * ND;
 #pragma unroll
-      for (int c = 0; c < ND; ++c) acc += (double)blk[c] * x_load(xc + c);
+      for (int c = 0; c < ND; ++c) {
+        double xcv = x_load(xc + c);
+        while (ilu_is_pending(xcv)) {
+          if (++polls > max_polls) {
+            __hip_atomic_store(&sync[ILU_SYNC_ERR], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+          xcv = x_load(xc + c);
+        }
+        acc += (double)blk[c] * xcv;
+      }
     }
     tp[slot][r][l] = acc;
   }
@@ -654,12 +689,10 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
     float y = 0.0f;
 #pragma unroll
     for (int k = 0; k < LPR; ++k) y += yp[slot][r][k];
-    x_store(x + e * N + r, forward ? (double)y : bval - (double)y);
+    x_store(out + e * N + r, forward ? (double)y : bval - (double)y);
+    // this thread read the row of `in` (which `reset` may alias) above
+    x_store(reset + e * N + r, __longlong_as_double((long long)ILU_PENDING));
   }
-  // this thread's stores have completed before the element's flag is set
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
-  if (act && tt == 0) __hip_atomic_store(&done[e], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // lanes per row of the sweeps: 1 for diagonal-E operators (a row meets at
@@ -754,12 +787,14 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
   p->h_fl = fl;
   p->h_bl = bl;
   p->blk_per_el = (double)(J->nnzb - (int64_t)nel * c->Np * c->Np) / 2.0 / std::max(1, nel);
-  if (hipMalloc((void**)&p->d_done, std::max(1, nel) * sizeof(int32_t)) != hipSuccess ||
+  const int64_t ny = std::max<int64_t>(1, c->numDof);
+  if (hipMalloc((void**)&p->d_y, ny * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&p->d_sync, ILU_SYNC_WORDS * sizeof(int32_t)) != hipSuccess) {
-    set_error("device allocation of the ILU sweep flags failed");
+    set_error("device allocation of the ILU sweep vector failed");
     return PDES_ERR_NOMEM;
   }
-  PDES_HIP(hipMemsetAsync(p->d_done, 0, std::max(1, nel) * sizeof(int32_t), c->stream));
+  hipLaunchKernelGGL(ilu_pending_fill, dim3((unsigned)ceil_div(ny, 256)), dim3(256), 0, c->stream, p->d_y, ny);
+  PDES_HIP(hipGetLastError());
   PDES_HIP(hipMemsetAsync(p->d_sync, 0, ILU_SYNC_WORDS * sizeof(int32_t), c->stream));
   // PDES_ILU_WAIT_POLLS: a smaller bound on the dependency waits (for tests
   // of the give-up path; the default bound is never reached by a sweep that
@@ -780,12 +815,12 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
 // level of its critical path, ILU_DEP_TICKET_US per chunk, and streams at
 // ILU_DEP_BW (measured on MI355X:
 // 2D diagonal-E p=2 sweeps over 246 levels of 123 elements take 4.8 us per
-// level launch and 2.6 us per dependency step; 3D Omega p=2 sweeps over 96
+// level launch and 1.85 us per dependency step; 3D Omega p=2 sweeps over 96
 // levels of 864 tets stream 2.1 TB/s level by level, 1.7 TB/s dependency-
 // driven).  The elements and levels are those of the instances being solved
 // when the batched solver says so, else the whole mesh.
 constexpr double ILU_LEVEL_FLOOR_US = 4.5, ILU_LEVEL_BW = 2.1e6;   // bytes per us
-constexpr double ILU_DEP_STEP_US = 2.6, ILU_DEP_BW = 1.7e6;
+constexpr double ILU_DEP_STEP_US = 1.85, ILU_DEP_BW = 1.7e6;
 // and one ticket (a device-scope atomic on one word) per chunk: about 21 ns
 // each, measured as the per-element rate of one-element chunks
 constexpr double ILU_DEP_TICKET_US = 0.021;
@@ -934,17 +969,21 @@ static int ilu_apply(pdes_ctx* c, const double* b, double* x) {
       const int32_t* chunks = sel ? (pass == 0 ? p->d_fsel : p->d_bsel) : nullptr;
       const int64_t nch = sel ? (pass == 0 ? p->sel_nf : p->sel_nb) : (pass == 0 ? p->fpad_n : p->bpad_n);
       if (nch == 0) continue;
+      // forward: b -> y, x set pending; backward: y -> x, y set pending again
+      const double* in = pass == 0 ? b : p->d_y;
+      double* out = pass == 0 ? p->d_y : x;
+      double* reset = pass == 0 ? x : p->d_y;
       hipLaunchKernelGGL(ilu_sweep_begin, dim3(1), dim3(1), 0, c->stream, p->d_sync);
       if (j->fp64)
         hipLaunchKernelGGL((ilu_sweep_dep_kernel<NP, ND, NS, DIAGE, double, LPR, CH>), dim3((unsigned)nch),
                            dim3(CH * N * LPR), 0, c->stream, order, chunks, nch, j->d_rowptr, j->d_colind, p->d_own,
-                           (const double*)j->d_vals, p->d_inv, b, x, pass == 0, c->d_el_inst, c->d_mask,
-                           c->d_inst_active, c->d_elface, c->nfaces, p->d_done, p->d_sync);
+                           (const double*)j->d_vals, p->d_inv, in, out, reset, pass == 0, c->d_el_inst, c->d_mask,
+                           c->d_inst_active, p->d_sync);
       else
         hipLaunchKernelGGL((ilu_sweep_dep_kernel<NP, ND, NS, DIAGE, float, LPR, CH>), dim3((unsigned)nch),
                            dim3(CH * N * LPR), 0, c->stream, order, chunks, nch, j->d_rowptr, j->d_colind, p->d_own,
-                           (const float*)j->d_vals, p->d_inv, b, x, pass == 0, c->d_el_inst, c->d_mask,
-                           c->d_inst_active, c->d_elface, c->nfaces, p->d_done, p->d_sync);
+                           (const float*)j->d_vals, p->d_inv, in, out, reset, pass == 0, c->d_el_inst, c->d_mask,
+                           c->d_inst_active, p->d_sync);
     }
     PDES_HIP(hipGetLastError());
     return PDES_OK;

@@ -402,11 +402,11 @@ DEP_CASES = [("vortex", 6, 2, "diage", "ir", False), ("vortex", 5, 3, "diage", "
 @pytest.mark.parametrize("kind,n,p,opk,flux,fp64", DEP_CASES)
 def test_ilu0_dependency_sweeps_match_level_sweeps(monkeypatch, kind, n, p, opk, flux, fp64):
     """The dependency-driven ILU(0) sweeps (one launch per triangle, elements
-    in ticket order, each waiting for its lower or upper neighbours'
-    completion flags; forced with PDES_ILU_DEP) give the bits of the
-    level-by-level sweeps (forced with PDES_ILU_LEVELS), over repeated
-    applications (the flags' epochs advance, nothing is cleared) with other
-    device work in between, for one-lane rows (diagonal-E) and several lanes
+    in ticket order, each loading its lower or upper neighbours' unknowns
+    until none holds the pending sentinel; forced with PDES_ILU_DEP) give the
+    bits of the level-by-level sweeps (forced with PDES_ILU_LEVELS), over
+    repeated applications (the sweep vector returns to the sentinel after
+    each, nothing is cleared) with other device work in between, for one-lane rows (diagonal-E) and several lanes
     per row (dense faces), FP32 and FP64 blocks, up to 128 x 128 x 2
     triangles.  Unforced, a cost model picks one of the two per application,
     so either choice gives the same result."""
