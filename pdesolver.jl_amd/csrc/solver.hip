@@ -609,6 +609,7 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
       for (int c = 0; c < ND; ++c) pre[u][c] = p < q1 ? vals[p * ND * ND + a * ND + c] : BT(0);
     }
   }
+  double acc = 0.0;
   if (act) {
     const int a = r % ND;
     // the neighbours' values: all loads issued at once, then the pending
@@ -638,7 +639,6 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
         for (int c = 0; c < ND; ++c)
           if (ilu_is_pending(xv[u][c])) xv[u][c] = x_load(out + (int64_t)pcol[u] * ND + c);
     }
-    double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < MAXB; ++u) {
       if (q0 + l + (int64_t)u * LPR < q1) {
@@ -664,34 +664,58 @@ ilu_sweep_dep_kernel(const int32_t* __restrict__ order, const int32_t* __restric
         acc += (double)blk[c] * xcv;
       }
     }
-    tp[slot][r][l] = acc;
+    if constexpr (LPR > 1) tp[slot][r][l] = acc;
   }
-  __syncthreads();
-  if (act && l == 0) {
-    double sum = 0.0;
+  if constexpr (LPR == 1) {
+    // one lane per row: the row sums need no reduction over lanes, so one
+    // barrier (for the rows of v) instead of three; the sums keep the
+    // lane-reduced form 0 + acc, 0 + y, so the bits are the same
+    if (act) {
+      const double sum = 0.0 + acc;
+      v[slot][r] = (float)(forward ? bval - sum : sum);
+    }
+    __syncthreads();
+    if (act) {
+      float y = 0.0f;
 #pragma unroll
-    for (int k = 0; k < LPR; ++k) sum += tp[slot][r][k];
-    v[slot][r] = (float)(forward ? bval - sum : sum);
-  }
-  __syncthreads();
-  if (act) {
-    float y = 0.0f;
+      for (int k = 0; k < PF; ++k)
+        if (k < N) y += mrow[k] * v[slot][k];
 #pragma unroll
-    for (int k = 0; k < PF; ++k)
-      if (l + k * LPR < N) y += mrow[k] * v[slot][l + k * LPR];
+      for (int k = PF; k < NC; ++k)
+        if (k < N) y += Mi[k * N + r] * v[slot][k];
+      const float ys = 0.0f + y;
+      x_store(out + e * N + r, forward ? (double)ys : bval - (double)ys);
+      // this thread read the row of `in` (which `reset` may alias) above
+      x_store(reset + e * N + r, __longlong_as_double((long long)ILU_PENDING));
+    }
+  } else {
+    __syncthreads();
+    if (act && l == 0) {
+      double sum = 0.0;
 #pragma unroll
-    for (int k = PF; k < NC; ++k)
-      if (l + k * LPR < N) y += Mi[(l + k * LPR) * N + r] * v[slot][l + k * LPR];
-    yp[slot][r][l] = y;
-  }
-  __syncthreads();
-  if (act && l == 0) {
-    float y = 0.0f;
+      for (int k = 0; k < LPR; ++k) sum += tp[slot][r][k];
+      v[slot][r] = (float)(forward ? bval - sum : sum);
+    }
+    __syncthreads();
+    if (act) {
+      float y = 0.0f;
 #pragma unroll
-    for (int k = 0; k < LPR; ++k) y += yp[slot][r][k];
-    x_store(out + e * N + r, forward ? (double)y : bval - (double)y);
-    // this thread read the row of `in` (which `reset` may alias) above
-    x_store(reset + e * N + r, __longlong_as_double((long long)ILU_PENDING));
+      for (int k = 0; k < PF; ++k)
+        if (l + k * LPR < N) y += mrow[k] * v[slot][l + k * LPR];
+#pragma unroll
+      for (int k = PF; k < NC; ++k)
+        if (l + k * LPR < N) y += Mi[(l + k * LPR) * N + r] * v[slot][l + k * LPR];
+      yp[slot][r][l] = y;
+    }
+    __syncthreads();
+    if (act && l == 0) {
+      float y = 0.0f;
+#pragma unroll
+      for (int k = 0; k < LPR; ++k) y += yp[slot][r][k];
+      x_store(out + e * N + r, forward ? (double)y : bval - (double)y);
+      // this thread read the row of `in` (which `reset` may alias) above
+      x_store(reset + e * N + r, __longlong_as_double((long long)ILU_PENDING));
+    }
   }
 }
 
@@ -815,12 +839,12 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
 // level of its critical path, ILU_DEP_TICKET_US per chunk, and streams at
 // ILU_DEP_BW (measured on MI355X:
 // 2D diagonal-E p=2 sweeps over 246 levels of 123 elements take 4.8 us per
-// level launch and 1.85 us per dependency step; 3D Omega p=2 sweeps over 96
+// level launch and 1.7 us per dependency step; 3D Omega p=2 sweeps over 96
 // levels of 864 tets stream 2.1 TB/s level by level, 1.7 TB/s dependency-
 // driven).  The elements and levels are those of the instances being solved
 // when the batched solver says so, else the whole mesh.
 constexpr double ILU_LEVEL_FLOOR_US = 4.5, ILU_LEVEL_BW = 2.1e6;   // bytes per us
-constexpr double ILU_DEP_STEP_US = 1.85, ILU_DEP_BW = 1.7e6;
+constexpr double ILU_DEP_STEP_US = 1.7, ILU_DEP_BW = 1.7e6;
 // and one ticket (a device-scope atomic on one word) per chunk: about 21 ns
 // each, measured as the per-element rate of one-element chunks
 constexpr double ILU_DEP_TICKET_US = 0.021;
