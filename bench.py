@@ -22,10 +22,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 # The batched solves run their 5 operator groups concurrently, each on its own
 # stream, and the served requests a stream per lane group; HIP maps a process's
 # streams onto GPU_MAX_HW_QUEUES hardware queues and streams sharing a queue
-# run in turn.  Give the process at least 8 queues (set before the first GPU
+# run in turn.  Give the process at least 16 queues (set before the first GPU
 # call, which HIP reads it at); the residual timing uses one stream either way.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 for _p in (os.path.join(ROOT, "pdesolver.jl_amd"), os.path.join(ROOT, "oracle")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
