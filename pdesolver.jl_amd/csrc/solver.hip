@@ -495,7 +495,7 @@ ilu_sweep_level_ml_kernel(const int32_t* __restrict__ lev_el, int64_t count, con
 // forward sweep writes y = L^-1 b into d_y and sets its elements' rows of x
 // pending (after reading b: x may be b); the backward sweep reads its own y,
 // writes x and sets its rows of y pending again, so d_y is pending between
-// applications (it starts so).  A wait gives up after a bounded number of
+// applications (each factorization sets it so).  A wait gives up after a bounded number of
 // polls, sets the error word and carries on, so every launch ends.
 // ---------------------------------------------------------------------------
 constexpr int ILU_WAIT_POLLS = 1 << 18;
@@ -812,13 +812,12 @@ static int ilu_structure(pdes_ctx* c, pdes_precond* p) {
   p->h_bl = bl;
   p->blk_per_el = (double)(J->nnzb - (int64_t)nel * c->Np * c->Np) / 2.0 / std::max(1, nel);
   const int64_t ny = std::max<int64_t>(1, c->numDof);
+  // d_y is set pending with each factorization (pc_setup)
   if (hipMalloc((void**)&p->d_y, ny * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&p->d_sync, ILU_SYNC_WORDS * sizeof(int32_t)) != hipSuccess) {
     set_error("device allocation of the ILU sweep vector failed");
     return PDES_ERR_NOMEM;
   }
-  hipLaunchKernelGGL(ilu_pending_fill, dim3((unsigned)ceil_div(ny, 256)), dim3(256), 0, c->stream, p->d_y, ny);
-  PDES_HIP(hipGetLastError());
   PDES_HIP(hipMemsetAsync(p->d_sync, 0, ILU_SYNC_WORDS * sizeof(int32_t), c->stream));
   // PDES_ILU_WAIT_POLLS: a smaller bound on the dependency waits (for tests
   // of the give-up path; the default bound is never reached by a sweep that
@@ -1253,6 +1252,15 @@ int pc_setup(pdes_ctx* c, int kind) {
   if (kind == PDES_PC_ILU0) {
     if (!p->ilu_ready && (st = ilu_structure(c, p)) != 0) return st;
     st = ilu_factor_dispatch(c);
+    // the dependency-driven sweeps' vector holds ILU_PENDING between
+    // applications; set it again with each factorization, so that an
+    // application cut short between its two sweeps cannot leave values
+    // behind for the next factorization's sweeps
+    if (!st) {
+      const int64_t ny = std::max<int64_t>(1, c->numDof);
+      hipLaunchKernelGGL(ilu_pending_fill, dim3((unsigned)ceil_div(ny, 256)), dim3(256), 0, c->stream, p->d_y, ny);
+      PDES_HIP(hipGetLastError());
+    }
   } else {
     st = pc_factor(c);
   }
