@@ -27,3 +27,22 @@ def test_binding_covers_header():
     declared = set(declared_symbols())
     bound = set(_lib.exported_symbols())
     assert declared <= bound, f"unbound: {sorted(declared - bound)}"
+
+
+def test_package_default_hardware_queue_count():
+    """Importing the package gives HIP 16 hardware queues where the
+    environment sets none (a batch runs a stream per operator group) and
+    leaves an explicit setting alone; run in child processes, so this
+    process's environment is untouched."""
+    import subprocess
+    import sys
+    code = "import os; from pdes_amd import _lib; print(os.environ['GPU_MAX_HW_QUEUES'])"
+    pkg = os.path.join(ROOT, "pdesolver.jl_amd")
+    for given, expected in ((None, "16"), ("4", "4")):
+        env = {k: v for k, v in os.environ.items() if k != "GPU_MAX_HW_QUEUES"}
+        if given is not None:
+            env["GPU_MAX_HW_QUEUES"] = given
+        env["PYTHONPATH"] = pkg + os.pathsep + env.get("PYTHONPATH", "")
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert r.stdout.strip() == expected
