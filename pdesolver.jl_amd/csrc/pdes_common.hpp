@@ -53,6 +53,17 @@ struct dual {
 template <int N> __device__ __forceinline__ double re(const dual<N>& a) { return a.v; }
 __device__ __forceinline__ double re(double a) { return a; }
 
+// The batch that workgroup g of nb works on, so that each XCD walks a
+// contiguous eighth of the batches: the hardware deals workgroups to the 8
+// XCDs round-robin (workgroup g runs on XCD g % 8), and XCD x takes batches
+// [x q + min(x, r), ...) of nb = 8 q + r.  Neighbouring batches, whose data
+// overlap (an element's face neighbours), are then read through the same L2.
+// A bijection on [0, nb).
+__device__ __forceinline__ int64_t xcd_contiguous_batch(int64_t g, int64_t nb) {
+  const int64_t q = nb / 8, r = nb % 8, x = g % 8;
+  return x * q + (x < r ? x : r) + g / 8;
+}
+
 // 1/x from the hardware reciprocal estimate (v_rcp_f64) refined by two
 // Newton-Raphson steps r <- r + r (1 - x r), each of which squares the
 // relative error.  Within about one ulp, not correctly rounded; meant for the

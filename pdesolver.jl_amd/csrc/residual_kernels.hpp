@@ -300,14 +300,10 @@ euler_residual_kernel(DevMesh mesh, DevSBP sbp, PhysConst phys,
   const int me = el * NP;
   // face contributions, planes [c][element][face][face node]
   T* const gpl = Smem::G_IN_F ? &sm.F[0][0] : Smem::G_IN_C ? &sm.C[Smem::G_C_BUF][0][0] : &sm.G[0][0][0][0];
-  // consecutive element batches on one XCD: the hardware deals workgroups to
-  // the 8 XCDs round-robin (workgroup g runs on XCD g % 8), so XCD x takes
-  // the contiguous range of batches [x q + min(x, r), ...) of nb = 8 q + r,
-  // and an element's face neighbours, in the adjacent batches or a mesh row
-  // away, were most likely just read into the same L2
-  const int64_t nb = gridDim.x, g = blockIdx.x, xq = nb / 8, xr = nb % 8, xcd = g % 8;
-  const int64_t batch = xcd * xq + (xcd < xr ? xcd : xr) + g / 8;
-  const int64_t eslot = batch * EPB + el;
+  // consecutive element batches on one XCD, so that an element's face
+  // neighbours (the adjacent batches, or a mesh row away) were most likely
+  // just read into the same L2
+  const int64_t eslot = xcd_contiguous_batch(blockIdx.x, gridDim.x) * EPB + el;
   const bool active = (el < EPB) && (eslot < launch_count(mesh));
   const int64_t e = active ? element_of(mesh, eslot) : 0;
 
