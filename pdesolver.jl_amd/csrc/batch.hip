@@ -283,16 +283,7 @@ __global__ void gmres_init(int nlist, const int32_t* __restrict__ ilist, int32_t
   coef[(int64_t)inst * mstride] = 0.0;
 }
 
-// y = x, y = a - b, y += x   over the active dofs of the instances with mask 1
-__global__ void copy_masked(const double* __restrict__ x, double* __restrict__ y, const int32_t* __restrict__ el_list,
-                            const int32_t* __restrict__ el_inst, int dpe, int64_t n, const int32_t* __restrict__ mask) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
-    const int64_t k = active_dof(el_list, t, dpe);
-    if (mask[el_inst[k / dpe]]) y[k] = x[k];
-  }
-}
-
+// y = a - b, y += x   over the active dofs of the instances with mask 1
 __global__ void sub_masked(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ y,
                            const int32_t* __restrict__ el_list, const int32_t* __restrict__ el_inst, int dpe,
                            int64_t n, const int32_t* __restrict__ mask) {
