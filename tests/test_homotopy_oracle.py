@@ -11,7 +11,7 @@ wface lambda / 2."""
 import numpy as np
 import pytest
 
-from pdes_amd import abi, problems
+from pdes_amd import problems
 from helpers import setup, state_for
 
 CASES = [("vortex", 3, 1, "omega"), ("vortex", 3, 2, "omega"), ("vortex", 2, 3, "diage"), ("cube", 1, 2, "omega")]

@@ -1,7 +1,6 @@
 """SBP operator identities (the properties SummationByParts.jl operators
 satisfy; the reference's own tests rely on them, e.g. test_flux.jl:191-200
 split-form zero element sum, test_dg.jl:112-131 uniform flow)."""
-import itertools
 
 import numpy as np
 import pytest
